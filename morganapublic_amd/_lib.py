@@ -1,0 +1,370 @@
+"""ctypes binding of libmgfe.so (declared in include/mgfe.h).
+
+The library is the product: there is no Python or CPU fallback.  Loading fails
+loudly (MgfeError) when libmgfe.so is missing, and every call that returns a
+non-zero status raises MgfeError with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmgfe.so")
+
+MG_OK = 0
+MG_ERR_ARG = -1
+MG_ERR_HIP = -2
+MG_ERR_STATE = -3
+MG_ERR_MESH = -4
+MG_ERR_NOCONV = -5
+MG_ERR_BREAKDOWN = -6
+MG_ERR_SIZE = -7
+
+SYS_FULL = 0
+SYS_INTERIOR = 1
+TERM_STIFF = 1
+TERM_MASS = 2
+COEFF_CONST = 0
+COEFF_P0 = 1
+FIELD_STIFF = 0
+FIELD_MASS = 1
+PREC_F64 = 0
+PREC_F32 = 1
+SOLVE_F64 = 0
+SOLVE_F32_REFINE = 1
+
+# every symbol include/mgfe.h declares
+EXPORTS = (
+    "mg_ctx_create", "mg_ctx_destroy", "mg_last_error", "mg_ctx_stream", "mg_ctx_synchronize",
+    "mg_mesh_set", "mg_field_set", "mg_field_set_device", "mg_dofmap_build", "mg_elem_dofs_get",
+    "mg_edges_get", "mg_dirichlet_set", "mg_dirichlet_mask_get", "mg_pattern_build",
+    "mg_pattern_get", "mg_slot_map_get", "mg_assemble", "mg_values_get", "mg_values_device",
+    "mg_spmv", "mg_spmv_device", "mg_pcg_solve", "mg_pcg_solve_device", "mg_adjoint_grad_k",
+    "mg_adjoint_grad_k_device", "mg_dual_fv_build", "mg_dual_fv_get", "mg_dd_residual",
+    "mg_dd_vjp", "mg_bernoulli",
+)
+
+
+class MgfeError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__("mgfe error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Operator(C.Structure):
+    _fields_ = [("terms", C.c_int), ("stiff_coeff", C.c_int), ("stiff_const", C.c_double),
+                ("stiff_rule", C.c_int), ("mass_coeff", C.c_int), ("mass_const", C.c_double),
+                ("mass_rule", C.c_int)]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("mode", C.c_int), ("tol", C.c_double), ("inner_tol", C.c_double),
+                ("max_iters", C.c_int), ("max_outer", C.c_int), ("check_every", C.c_int),
+                ("use_x0", C.c_int)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("iters", C.c_int), ("outer", C.c_int),
+                ("relres", C.c_double), ("solve_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {"status": self.status, "iters": self.iters, "outer": self.outer,
+                "relres": self.relres, "solve_ms": self.solve_ms}
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads libmgfe.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MgfeError(MG_ERR_STATE, "libmgfe.so not built (run __graft_entry__.build() or "
+                                          "make -C morganapublic_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+_P = C.c_void_p
+_I = C.c_int
+_I64 = C.c_int64
+_PI64 = C.POINTER(C.c_int64)
+
+
+def _declare(L):
+    sig = {
+        "mg_ctx_create": [_I, C.POINTER(_P)],
+        "mg_ctx_destroy": [_P],
+        "mg_ctx_synchronize": [_P],
+        "mg_mesh_set": [_P, _I64, _P, _I64, _P, _I64, _P, _P],
+        "mg_field_set": [_P, _I, _P],
+        "mg_field_set_device": [_P, _I, _P],
+        "mg_dofmap_build": [_P, _I, _PI64, _PI64],
+        "mg_elem_dofs_get": [_P, _P],
+        "mg_edges_get": [_P, _P, _P],
+        "mg_dirichlet_set": [_P, _I, _P, _PI64],
+        "mg_dirichlet_mask_get": [_P, _P],
+        "mg_pattern_build": [_P, _I, _PI64, _PI64],
+        "mg_pattern_get": [_P, _I, _P, _P],
+        "mg_slot_map_get": [_P, _I, _P],
+        "mg_assemble": [_P, _I, C.POINTER(Operator), _I],
+        "mg_values_get": [_P, _I, _I, _P],
+        "mg_values_device": [_P, _I, _I, C.POINTER(_P)],
+        "mg_spmv": [_P, _I, _I, _P, _P],
+        "mg_spmv_device": [_P, _I, _I, _P, _P],
+        "mg_pcg_solve": [_P, C.POINTER(SolverParams), _P, _P, C.POINTER(SolveInfo)],
+        "mg_pcg_solve_device": [_P, C.POINTER(SolverParams), _P, _P, C.POINTER(SolveInfo)],
+        "mg_adjoint_grad_k": [_P, _I, _I, _P, _P, _P],
+        "mg_adjoint_grad_k_device": [_P, _I, _I, _P, _P, _P],
+        "mg_dual_fv_build": [_P],
+        "mg_dual_fv_get": [_P, _P, _P, _P, _P],
+        "mg_dd_residual": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+        "mg_dd_vjp": [_P, _P, _P, _P, _P, _P, _P, _P, _P],
+        "mg_bernoulli": [_P, _I64, _P, _P],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = _I
+    L.mg_last_error.argtypes = [_P]
+    L.mg_last_error.restype = C.c_char_p
+    L.mg_ctx_stream.argtypes = [_P]
+    L.mg_ctx_stream.restype = _P
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def _arr(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+class Context:
+    """One libmgfe context: one GPU, one HIP stream, device-resident mesh,
+    dof map, patterns, matrices and solver workspace."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = _P()
+        rc = self._L.mg_ctx_create(device, C.byref(h))
+        if rc != MG_OK:
+            raise MgfeError(rc, "mg_ctx_create(%d) failed (no GPU?)" % device)
+        self.h = h
+        self.order = None
+        self.nT = self.nV = 0
+        self.ndofs = self.nE = self.nfree = 0
+        self._nnz = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.mg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != MG_OK:
+            raise MgfeError(rc, self._L.mg_last_error(self.h).decode())
+
+    @property
+    def stream(self) -> int:
+        return self._L.mg_ctx_stream(self.h)
+
+    def synchronize(self):
+        self._chk(self._L.mg_ctx_synchronize(self.h))
+
+    # -- mesh / fields ---------------------------------------------------------
+    def mesh_set(self, xyz, tets, btri=None, bgeo=None):
+        xyz, tets = _arr(xyz, np.float64), _arr(tets, np.int32)
+        btri = _arr(btri if btri is not None else np.zeros((0, 3)), np.int32)
+        bgeo = _arr(bgeo if bgeo is not None else np.zeros(0), np.int32)
+        self._keep = (xyz, tets, btri, bgeo)
+        self._chk(self._L.mg_mesh_set(self.h, xyz.shape[0], _ptr(xyz), tets.shape[0], _ptr(tets),
+                                      btri.shape[0], _ptr(btri), _ptr(bgeo)))
+        self.nV, self.nT = xyz.shape[0], tets.shape[0]
+
+    def field_set(self, which: int, values):
+        v = _arr(values, np.float64)
+        if v.size != self.nT:
+            raise ValueError("field needs one value per element")
+        self._chk(self._L.mg_field_set(self.h, which, _ptr(v)))
+
+    def field_set_device(self, which: int, ptr: int):
+        self._chk(self._L.mg_field_set_device(self.h, which, _P(ptr)))
+
+    # -- dofs ------------------------------------------------------------------
+    def dofmap_build(self, order: int):
+        nd, ne = _I64(), _I64()
+        self._chk(self._L.mg_dofmap_build(self.h, order, C.byref(nd), C.byref(ne)))
+        self._nnz = {}
+        self.order, self.ndofs, self.nE = order, nd.value, ne.value
+        return self.ndofs
+
+    @property
+    def nloc(self) -> int:
+        return 4 if self.order == 1 else 10
+
+    def elem_dofs(self):
+        d = np.zeros((self.nT, self.nloc), dtype=np.int32)
+        self._chk(self._L.mg_elem_dofs_get(self.h, _ptr(d)))
+        return d
+
+    def edges(self):
+        en = np.zeros((self.nE, 2), dtype=np.int32)
+        ee = np.zeros((self.nT, 6), dtype=np.int32)
+        self._chk(self._L.mg_edges_get(self.h, _ptr(en), _ptr(ee)))
+        return ee, en
+
+    def dirichlet_set(self, geo_ids):
+        g = _arr(geo_ids, np.int32)
+        nf = _I64()
+        self._chk(self._L.mg_dirichlet_set(self.h, g.size, _ptr(g), C.byref(nf)))
+        self._nnz.pop(SYS_INTERIOR, None)
+        self.nfree = nf.value
+        return self.nfree
+
+    def dirichlet_mask(self):
+        m = np.zeros(self.ndofs, dtype=np.uint8)
+        self._chk(self._L.mg_dirichlet_mask_get(self.h, _ptr(m)))
+        return m
+
+    # -- pattern / values ------------------------------------------------------
+    def pattern_build(self, sys: int):
+        nr, nz = _I64(), _I64()
+        self._chk(self._L.mg_pattern_build(self.h, sys, C.byref(nr), C.byref(nz)))
+        self._nnz[sys] = (nr.value, nz.value)
+        return nr.value, nz.value
+
+    def pattern(self, sys: int):
+        nr, nz = self.sizes(sys)
+        rp = np.zeros(nr + 1, dtype=np.int32)
+        ci = np.zeros(nz, dtype=np.int32)
+        self._chk(self._L.mg_pattern_get(self.h, sys, _ptr(rp), _ptr(ci)))
+        return rp, ci
+
+    def sizes(self, sys: int):
+        if sys not in self._nnz:
+            raise MgfeError(MG_ERR_STATE, "pattern %d not built" % sys)
+        return self._nnz[sys]
+
+    def slot_map(self, sys: int):
+        s = np.zeros(self.nT * self.nloc * self.nloc, dtype=np.int32)
+        self._chk(self._L.mg_slot_map_get(self.h, sys, _ptr(s)))
+        return s.reshape(self.nT, self.nloc, self.nloc)
+
+    def assemble(self, sys: int, terms: int, stiff_coeff=COEFF_CONST, stiff_const=1.0,
+                 stiff_rule=4, mass_coeff=COEFF_CONST, mass_const=1.0, mass_rule=4,
+                 prec=PREC_F64):
+        op = Operator(terms, stiff_coeff, stiff_const, stiff_rule, mass_coeff, mass_const,
+                      mass_rule)
+        self._chk(self._L.mg_assemble(self.h, sys, C.byref(op), prec))
+
+    def values(self, sys: int, prec=PREC_F64):
+        _, nz = self.sizes(sys)
+        v = np.zeros(nz, dtype=np.float64 if prec == PREC_F64 else np.float32)
+        self._chk(self._L.mg_values_get(self.h, sys, prec, _ptr(v)))
+        return v
+
+    def values_device_ptr(self, sys: int, prec=PREC_F64) -> int:
+        p = _P()
+        self._chk(self._L.mg_values_device(self.h, sys, prec, C.byref(p)))
+        return p.value
+
+    # -- linear algebra ----------------------------------------------------------
+    def spmv(self, sys: int, x, prec=PREC_F64):
+        x = _arr(x, np.float64)
+        y = np.zeros_like(x)
+        self._chk(self._L.mg_spmv(self.h, sys, prec, _ptr(x), _ptr(y)))
+        return y
+
+    def spmv_device(self, sys: int, x_ptr: int, y_ptr: int, prec=PREC_F64):
+        self._chk(self._L.mg_spmv_device(self.h, sys, prec, _P(x_ptr), _P(y_ptr)))
+
+    @staticmethod
+    def _params(mode, tol, inner_tol, max_iters, max_outer, check_every, use_x0):
+        return SolverParams(mode, tol, inner_tol, max_iters, max_outer, check_every,
+                            1 if use_x0 else 0)
+
+    def pcg(self, b, mode=SOLVE_F64, tol=1e-10, inner_tol=1e-5, max_iters=100000, max_outer=30,
+            check_every=32, x0=None, raise_on_fail=True):
+        b = _arr(b, np.float64)
+        x = np.zeros_like(b) if x0 is None else _arr(x0, np.float64).copy()
+        prm = self._params(mode, tol, inner_tol, max_iters, max_outer, check_every, x0 is not None)
+        info = SolveInfo()
+        rc = self._L.mg_pcg_solve(self.h, C.byref(prm), _ptr(b), _ptr(x), C.byref(info))
+        if rc != MG_OK and raise_on_fail:
+            self._chk(rc)
+        return x, info.as_dict()
+
+    def pcg_device(self, b_ptr: int, x_ptr: int, mode=SOLVE_F64, tol=1e-10, inner_tol=1e-5,
+                   max_iters=100000, max_outer=30, check_every=32, use_x0=False,
+                   raise_on_fail=True):
+        prm = self._params(mode, tol, inner_tol, max_iters, max_outer, check_every, use_x0)
+        info = SolveInfo()
+        rc = self._L.mg_pcg_solve_device(self.h, C.byref(prm), _P(b_ptr), _P(x_ptr),
+                                         C.byref(info))
+        if rc != MG_OK and raise_on_fail:
+            self._chk(rc)
+        return info.as_dict()
+
+    # -- gradients ---------------------------------------------------------------
+    def adjoint_grad_k(self, sys: int, u, lam, rule=4):
+        u, lam = _arr(u, np.float64), _arr(lam, np.float64)
+        g = np.zeros(self.nT)
+        self._chk(self._L.mg_adjoint_grad_k(self.h, sys, rule, _ptr(u), _ptr(lam), _ptr(g)))
+        return g
+
+    def adjoint_grad_k_device(self, sys: int, u_ptr: int, lam_ptr: int, out_ptr: int, rule=4):
+        self._chk(self._L.mg_adjoint_grad_k_device(self.h, sys, rule, _P(u_ptr), _P(lam_ptr),
+                                                   _P(out_ptr)))
+
+    # -- dual mesh / SG ------------------------------------------------------------
+    def dual_fv_build(self):
+        self._chk(self._L.mg_dual_fv_build(self.h))
+
+    def dual_fv(self):
+        surf = np.zeros(self.nE)
+        nrm = np.zeros((self.nE, 3))
+        cvol = np.zeros(self.nV)
+        sig = np.zeros((self.nT, 6))
+        self._chk(self._L.mg_dual_fv_get(self.h, _ptr(surf), _ptr(nrm), _ptr(cvol), _ptr(sig)))
+        return {"edge_surf": surf, "edge_n": nrm, "cell_volume": cvol, "sigma_el": sig}
+
+    def dd_residual(self, eps, rho, mu, phi, dens, fixed):
+        a = [_arr(x, np.float64) for x in (eps, rho, mu, phi, dens)]
+        fx = _arr(fixed, np.uint8)
+        rp, rn = np.zeros(self.nV), np.zeros(self.nV)
+        loss = C.c_double()
+        self._chk(self._L.mg_dd_residual(self.h, *[_ptr(x) for x in a], _ptr(fx), _ptr(rp),
+                                         _ptr(rn), C.byref(loss)))
+        return loss.value, rp, rn
+
+    def dd_vjp(self, eps, rho, mu, phi, dens, fixed):
+        a = [_arr(x, np.float64) for x in (eps, rho, mu, phi, dens)]
+        fx = _arr(fixed, np.uint8)
+        ge, gm = np.zeros(self.nT), np.zeros(self.nT)
+        self._chk(self._L.mg_dd_vjp(self.h, *[_ptr(x) for x in a], _ptr(fx), _ptr(ge), _ptr(gm)))
+        return ge, gm
+
+    def bernoulli(self, x):
+        x = _arr(x, np.float64)
+        y = np.zeros_like(x)
+        self._chk(self._L.mg_bernoulli(self.h, x.size, _ptr(x), _ptr(y)))
+        return y

@@ -1,0 +1,489 @@
+// Barycentric dual-mesh data and the Scharfetter-Gummel drift-diffusion
+// residual with its FP64 vector-Jacobian products.
+//
+// Dual data (dual3d<linearTetra>::buildFiniteVolumeData, dual3d.hpp:606-1025):
+// per tet, the 12 interface triangles (edge midpoint, face barycentre, element
+// barycentre) have normals N = (P2-P1) x (P3-P1); they are summed per edge with
+// their sign aligned to the edge direction, edgeSurf = |edgeN| / 2, and each
+// vertex receives the 6 sub-cell volumes of every tet around it.  Sums over
+// elements are gathers over sorted incidence lists (deterministic).
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+
+#include "mg_internal.h"
+
+namespace mg {
+
+__constant__ int c_ev[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+__constant__ int c_fv[4][3] = {{0, 2, 1}, {0, 1, 3}, {1, 2, 3}, {0, 3, 2}};  // faceToPoint
+__constant__ int c_fe[4][3] = {{2, 1, 0}, {0, 4, 3}, {1, 5, 4}, {3, 5, 2}};  // faceToEdge
+
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// per element: aligned interface normal sum for each local edge, and the
+// sub-cell volume each local vertex receives
+__global__ void k_dual_elem(int64_t nT, const int32_t *__restrict__ tets,
+                            const double *__restrict__ xyz, double *__restrict__ ne,
+                            double *__restrict__ subvol) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nT) return;
+  int32_t tv[4];
+  double P[4][3];
+  for (int v = 0; v < 4; ++v) {
+    tv[v] = tets[4 * e + v];
+    for (int d = 0; d < 3; ++d) P[v][d] = xyz[3 * tv[v] + d];
+  }
+  double BA[3], F[4][3], Em[6][3], N[6][3], sv[4] = {0, 0, 0, 0};
+  for (int d = 0; d < 3; ++d) BA[d] = (P[0][d] + P[1][d] + P[2][d] + P[3][d]) / 4.0;
+  for (int f = 0; f < 4; ++f)
+    for (int d = 0; d < 3; ++d)
+      F[f][d] = (P[c_fv[f][0]][d] + P[c_fv[f][1]][d] + P[c_fv[f][2]][d]) / 3.0;
+  for (int j = 0; j < 6; ++j)
+    for (int d = 0; d < 3; ++d) {
+      Em[j][d] = (P[c_ev[j][0]][d] + P[c_ev[j][1]][d]) / 2.0;
+      N[j][d] = 0.0;
+    }
+  for (int f = 0; f < 4; ++f)
+    for (int k = 0; k < 3; ++k) {
+      const int j = c_fe[f][k];
+      double a[3], b[3], n[3], t[3];
+      for (int d = 0; d < 3; ++d) {
+        a[d] = F[f][d] - Em[j][d];
+        b[d] = BA[d] - Em[j][d];
+      }
+      cross3(a, b, n);
+      int lo = c_ev[j][0], hi = c_ev[j][1];
+      if (tv[lo] > tv[hi]) {
+        int s = lo;
+        lo = hi;
+        hi = s;
+      }
+      for (int d = 0; d < 3; ++d) t[d] = P[hi][d] - P[lo][d];
+      const double sgn = (n[0] * t[0] + n[1] * t[1] + n[2] * t[2]) >= 0.0 ? 1.0 : -1.0;
+      for (int d = 0; d < 3; ++d) N[j][d] += sgn * n[d];
+      for (int end = 0; end < 2; ++end) {
+        const int v = c_ev[j][end];
+        double m[3], cc[3];
+        for (int d = 0; d < 3; ++d) {
+          a[d] = Em[j][d] - BA[d];
+          b[d] = F[f][d] - BA[d];
+          cc[d] = P[v][d] - BA[d];
+        }
+        cross3(a, b, m);
+        sv[v] += fabs(m[0] * cc[0] + m[1] * cc[1] + m[2] * cc[2]) / 6.0;
+      }
+    }
+  for (int j = 0; j < 6; ++j)
+    for (int d = 0; d < 3; ++d) ne[18 * e + 3 * j + d] = N[j][d];
+  for (int v = 0; v < 4; ++v) subvol[4 * e + v] = sv[v];
+}
+
+__global__ void k_edge_gather(int64_t nE, const int32_t *__restrict__ ptr,
+                              const int32_t *__restrict__ inc, const double *__restrict__ ne,
+                              double *__restrict__ surf, double *__restrict__ nrm) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nE) return;
+  double s[3] = {0, 0, 0};
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+    const int32_t ej = inc[k];
+    for (int d = 0; d < 3; ++d) s[d] += ne[3 * (int64_t)ej + d];
+  }
+  const double nn = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+  surf[i] = 0.5 * nn;
+  for (int d = 0; d < 3; ++d) nrm[3 * i + d] = s[d] / nn;
+}
+
+__global__ void k_sigma(int64_t nT, const int32_t *__restrict__ elem_edge,
+                        const double *__restrict__ ne, const double *__restrict__ nrm,
+                        double *__restrict__ sigma) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 6 * nT) return;
+  const double *nh = nrm + 3 * (int64_t)elem_edge[i];
+  sigma[i] = 0.5 * (ne[3 * i] * nh[0] + ne[3 * i + 1] * nh[1] + ne[3 * i + 2] * nh[2]);
+}
+
+__global__ void k_vertex_gather(int64_t nV, const int32_t *__restrict__ ptr,
+                                const int32_t *__restrict__ inc, const double *__restrict__ subvol,
+                                double *__restrict__ cvol) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nV) return;
+  double s = 0.0;
+  for (int k = ptr[v]; k < ptr[v + 1]; ++k) {
+    const int32_t el = inc[k];  // e << 2 | l
+    s += subvol[4 * (int64_t)(el >> 2) + (el & 3)];
+  }
+  cvol[v] = s;
+}
+
+__global__ void k_keys_i32(int64_t n, const int32_t *__restrict__ key_src, int shift,
+                           int32_t *__restrict__ keys, int32_t *__restrict__ vals, int mode) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = key_src[i];
+  // mode 0: value = i; mode 1: value = (i / 4) << 2 | (i % 4)
+  vals[i] = mode == 0 ? (int32_t)i : (int32_t)(((i >> 2) << shift) | (i & 3));
+}
+
+__global__ void k_count_i32(int64_t n, const int32_t *__restrict__ keys, int32_t *__restrict__ cnt) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&cnt[keys[i]], 1);
+}
+
+// CSR of the items grouped by integer key (stable: original order inside a key)
+static void group_by(Ctx *c, const int32_t *keys_src, int64_t n, int64_t nkeys, int mode,
+                     int32_t **ptr_out, int32_t **inc_out) {
+  int32_t *k0 = dalloc<int32_t>(c, n), *k1 = dalloc<int32_t>(c, n);
+  int32_t *v0 = dalloc<int32_t>(c, n), *v1 = dalloc<int32_t>(c, n);
+  k_keys_i32<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, keys_src, 2, k0, v0, mode);
+  int bits = 1;
+  while ((1ll << bits) <= nkeys) ++bits;
+  size_t tb = 0;
+  MG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, (int)n, 0, bits,
+                                            c->stream));
+  void *t = scratch(c, tb);
+  MG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, k0, k1, v0, v1, (int)n, 0, bits, c->stream));
+  int32_t *cnt = dalloc<int32_t>(c, nkeys + 1);
+  MG_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (nkeys + 1), c->stream));
+  k_count_i32<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, k1, cnt);
+  int32_t *ptr = dalloc<int32_t>(c, nkeys + 1);
+  tb = 0;
+  MG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, ptr, (int)(nkeys + 1), c->stream));
+  t = scratch(c, tb);
+  MG_HIP(hipcub::DeviceScan::ExclusiveSum(t, tb, cnt, ptr, (int)(nkeys + 1), c->stream));
+  MG_HIP(hipGetLastError());
+  MG_HIP(hipStreamSynchronize(c->stream));
+  dfree(c, k0);
+  dfree(c, k1);
+  dfree(c, v0);
+  dfree(c, cnt);
+  *ptr_out = ptr;
+  *inc_out = v1;
+}
+
+static void build_vertex_incidence(Ctx *c) {
+  if (c->vinc_ptr) return;
+  group_by(c, c->tets, 4 * c->nT, c->nV, 1, &c->vinc_ptr, &c->vinc);
+}
+
+void build_dual(Ctx *c) {
+  MG_CHECK(c->edges_ok, MG_ERR_STATE, "dual: dof map (edges) not built");
+  const int64_t nT = c->nT, nE = c->nE, nV = c->nV;
+  double *ne = dalloc<double>(c, 18 * nT), *sv = dalloc<double>(c, 4 * nT);
+  k_dual_elem<<<blocks_for(nT, 128), 128, 0, c->stream>>>(nT, c->tets, c->xyz, ne, sv);
+  MG_HIP(hipGetLastError());
+  int32_t *eptr = nullptr, *einc = nullptr;
+  group_by(c, c->elem_edge, 6 * nT, nE, 0, &eptr, &einc);
+  build_vertex_incidence(c);
+  for (double *p : {c->edge_surf, c->edge_n, c->cell_volume, c->sigma_el}) dfree(c, p);
+  c->edge_surf = dalloc<double>(c, nE);
+  c->edge_n = dalloc<double>(c, 3 * nE);
+  c->cell_volume = dalloc<double>(c, nV);
+  c->sigma_el = dalloc<double>(c, 6 * nT);
+  k_edge_gather<<<blocks_for(nE, 256), 256, 0, c->stream>>>(nE, eptr, einc, ne, c->edge_surf,
+                                                          c->edge_n);
+  k_sigma<<<blocks_for(6 * nT, 256), 256, 0, c->stream>>>(nT, c->elem_edge, ne, c->edge_n,
+                                                        c->sigma_el);
+  k_vertex_gather<<<blocks_for(nV, 256), 256, 0, c->stream>>>(nV, c->vinc_ptr, c->vinc, sv,
+                                                            c->cell_volume);
+  MG_HIP(hipGetLastError());
+  MG_HIP(hipStreamSynchronize(c->stream));
+  for (void *p : {(void *)ne, (void *)sv, (void *)eptr, (void *)einc}) dfree(c, p);
+  c->dual_ok = true;
+}
+
+// ---- Scharfetter-Gummel -------------------------------------------------------------
+// B(x) = x / (e^x - 1) in FP64; Taylor series 1 - x/2 + x^2/12 - x^4/720 + x^6/30240
+// for |x| < 1e-2 (cancellation-free near 0, B(0) = 1).
+__device__ __forceinline__ double bernoulli(double x) {
+  const double ax = fabs(x);
+  if (ax < 1e-2) {
+    const double x2 = x * x;
+    return 1.0 - 0.5 * x + x2 / 12.0 * (1.0 - x2 / 60.0 * (1.0 - x2 / 42.0));
+  }
+  if (x > 700.0) return x * exp(-x);
+  return x / expm1(x);
+}
+
+__global__ void k_bernoulli(int64_t n, const double *__restrict__ x, double *__restrict__ y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = bernoulli(x[i]);
+}
+
+// P1 geometry: grad lambda (4x3) and volume
+__device__ __forceinline__ double p1_grads(const double *__restrict__ xyz, const int32_t *id,
+                                           double G[4][3]) {
+  double T[3][3];
+  for (int cc = 0; cc < 3; ++cc)
+    for (int r = 0; r < 3; ++r) T[r][cc] = xyz[3 * id[cc + 1] + r] - xyz[3 * id[0] + r];
+  const double d = T[0][0] * T[1][1] * T[2][2] - T[0][0] * T[1][2] * T[2][1] -
+                   T[1][0] * T[0][1] * T[2][2] + T[1][0] * T[0][2] * T[2][1] +
+                   T[2][0] * T[0][1] * T[1][2] - T[2][0] * T[0][2] * T[1][1];
+  G[1][0] = (T[1][1] * T[2][2] - T[1][2] * T[2][1]) / d;
+  G[1][1] = -(T[0][1] * T[2][2] - T[0][2] * T[2][1]) / d;
+  G[1][2] = (T[0][1] * T[1][2] - T[0][2] * T[1][1]) / d;
+  G[2][0] = -(T[1][0] * T[2][2] - T[1][2] * T[2][0]) / d;
+  G[2][1] = (T[0][0] * T[2][2] - T[0][2] * T[2][0]) / d;
+  G[2][2] = -(T[0][0] * T[1][2] - T[0][2] * T[1][0]) / d;
+  G[3][0] = (T[1][0] * T[2][1] - T[1][1] * T[2][0]) / d;
+  G[3][1] = -(T[0][0] * T[2][1] - T[0][1] * T[2][0]) / d;
+  G[3][2] = (T[0][0] * T[1][1] - T[0][1] * T[1][0]) / d;
+  for (int k = 0; k < 3; ++k) G[0][k] = -(G[1][k] + G[2][k] + G[3][k]);
+  return d / 6.0;
+}
+
+__device__ __forceinline__ double edge_len(const double *__restrict__ xyz, int a, int b) {
+  double s = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    const double t = xyz[3 * a + d] - xyz[3 * b + d];
+    s += t * t;
+  }
+  return sqrt(s);
+}
+
+// SG flux of local edge j of element e, oriented from its first to its second
+// local vertex: mu sigma / L * (n_b B(phi_b - phi_a) - n_a B(phi_a - phi_b))
+__device__ __forceinline__ double sg_edge(const double *__restrict__ xyz, const int32_t *id, int j,
+                                          double coef, const double *__restrict__ phi,
+                                          const double *__restrict__ dens) {
+  const int a = id[c_ev[j][0]], b = id[c_ev[j][1]];
+  const double L = edge_len(xyz, a, b);
+  const double dab = phi[b] - phi[a];
+  return coef / L * (dens[b] * bernoulli(dab) - dens[a] * bernoulli(-dab));
+}
+
+// vertex-owner residual: each vertex gathers the contributions of its elements
+// in incidence order
+__global__ void k_dd_residual(int64_t nV, const int32_t *__restrict__ vptr,
+                              const int32_t *__restrict__ vinc, const int32_t *__restrict__ tets,
+                              const double *__restrict__ xyz, const double *__restrict__ sigma,
+                              const double *__restrict__ eps, const double *__restrict__ rho,
+                              const double *__restrict__ mu, const double *__restrict__ phi,
+                              const double *__restrict__ dens, const uint8_t *__restrict__ fixed,
+                              double *__restrict__ rphi, double *__restrict__ rn) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nV) return;
+  double sp = 0.0, sn = 0.0;
+  for (int k = vptr[v]; k < vptr[v + 1]; ++k) {
+    const int32_t el = vinc[k];
+    const int64_t e = el >> 2;
+    const int l = el & 3;
+    int32_t id[4];
+    for (int q = 0; q < 4; ++q) id[q] = tets[4 * e + q];
+    double G[4][3];
+    const double vol = p1_grads(xyz, id, G);
+    double gp[3] = {0, 0, 0};
+    for (int q = 0; q < 4; ++q)
+      for (int d = 0; d < 3; ++d) gp[d] += phi[id[q]] * G[q][d];
+    sp += eps[e] * vol * (G[l][0] * gp[0] + G[l][1] * gp[1] + G[l][2] * gp[2]) - rho[e] * vol / 4.0;
+    for (int j = 0; j < 6; ++j) {
+      const bool first = c_ev[j][0] == l, second = c_ev[j][1] == l;
+      if (!first && !second) continue;
+      const double F = sg_edge(xyz, id, j, mu[e] * sigma[6 * e + j], phi, dens);
+      sn += first ? F : -F;
+    }
+  }
+  const bool fx = fixed[v] != 0;
+  rphi[v] = fx ? 0.0 : sp;
+  rn[v] = fx ? 0.0 : sn;
+}
+
+__global__ void k_dd_vjp(int64_t nT, const int32_t *__restrict__ tets,
+                         const double *__restrict__ xyz, const double *__restrict__ sigma,
+                         const double *__restrict__ phi, const double *__restrict__ dens,
+                         const double *__restrict__ rphi, const double *__restrict__ rn,
+                         double *__restrict__ g_eps, double *__restrict__ g_mu) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nT) return;
+  int32_t id[4];
+  for (int q = 0; q < 4; ++q) id[q] = tets[4 * e + q];
+  double G[4][3];
+  const double vol = p1_grads(xyz, id, G);
+  double gp[3] = {0, 0, 0};
+  for (int q = 0; q < 4; ++q)
+    for (int d = 0; d < 3; ++d) gp[d] += phi[id[q]] * G[q][d];
+  double ge = 0.0;
+  for (int q = 0; q < 4; ++q)
+    ge += 2.0 * rphi[id[q]] * vol * (G[q][0] * gp[0] + G[q][1] * gp[1] + G[q][2] * gp[2]);
+  double gm = 0.0;
+  for (int j = 0; j < 6; ++j) {
+    const int a = id[c_ev[j][0]], b = id[c_ev[j][1]];
+    const double f = sg_edge(xyz, id, j, sigma[6 * e + j], phi, dens);
+    gm += 2.0 * (rn[a] - rn[b]) * f;
+  }
+  g_eps[e] = ge;
+  g_mu[e] = gm;
+}
+
+__global__ void k_loss(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
+                       double *__restrict__ partial) {
+  __shared__ double sh[4];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += a[i] * a[i] + b[i] * b[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+struct DDBuffers {
+  double *eps, *rho, *mu, *phi, *dens, *rphi, *rn, *ge, *gm, *partial;
+  uint8_t *fixed;
+};
+
+static DDBuffers dd_upload(Ctx *c, const double *eps, const double *rho, const double *mu,
+                           const double *phi, const double *dens, const uint8_t *fixed) {
+  DDBuffers B;
+  const int64_t nT = c->nT, nV = c->nV;
+  B.eps = dalloc<double>(c, nT);
+  B.rho = dalloc<double>(c, nT);
+  B.mu = dalloc<double>(c, nT);
+  B.ge = dalloc<double>(c, nT);
+  B.gm = dalloc<double>(c, nT);
+  B.phi = dalloc<double>(c, nV);
+  B.dens = dalloc<double>(c, nV);
+  B.rphi = dalloc<double>(c, nV);
+  B.rn = dalloc<double>(c, nV);
+  B.partial = dalloc<double>(c, 1024);
+  B.fixed = dalloc<uint8_t>(c, nV);
+  MG_HIP(hipMemcpyAsync(B.eps, eps, sizeof(double) * nT, hipMemcpyHostToDevice, c->stream));
+  MG_HIP(hipMemcpyAsync(B.rho, rho, sizeof(double) * nT, hipMemcpyHostToDevice, c->stream));
+  MG_HIP(hipMemcpyAsync(B.mu, mu, sizeof(double) * nT, hipMemcpyHostToDevice, c->stream));
+  MG_HIP(hipMemcpyAsync(B.phi, phi, sizeof(double) * nV, hipMemcpyHostToDevice, c->stream));
+  MG_HIP(hipMemcpyAsync(B.dens, dens, sizeof(double) * nV, hipMemcpyHostToDevice, c->stream));
+  MG_HIP(hipMemcpyAsync(B.fixed, fixed, nV, hipMemcpyHostToDevice, c->stream));
+  return B;
+}
+
+static void dd_free(Ctx *c, DDBuffers &B) {
+  for (void *p : {(void *)B.eps, (void *)B.rho, (void *)B.mu, (void *)B.ge, (void *)B.gm,
+                  (void *)B.phi, (void *)B.dens, (void *)B.rphi, (void *)B.rn, (void *)B.partial,
+                  (void *)B.fixed})
+    dfree(c, p);
+}
+
+static void dd_residual_launch(Ctx *c, DDBuffers &B) {
+  k_dd_residual<<<blocks_for(c->nV, 128), 128, 0, c->stream>>>(
+      c->nV, c->vinc_ptr, c->vinc, c->tets, c->xyz, c->sigma_el, B.eps, B.rho, B.mu, B.phi, B.dens,
+      B.fixed, B.rphi, B.rn);
+  MG_HIP(hipGetLastError());
+}
+
+}  // namespace mg
+
+using namespace mg;
+
+#define API_BEGIN(c)           \
+  if (!(c)) return MG_ERR_ARG; \
+  try {                        \
+    MG_HIP(hipSetDevice((c)->device));
+#define API_END(c)                  \
+  }                                 \
+  catch (const mg::Error &e) {      \
+    (c)->err = e.msg;               \
+    return e.code;                  \
+  }                                 \
+  catch (const std::exception &e) { \
+    (c)->err = e.what();            \
+    return MG_ERR_HIP;              \
+  }                                 \
+  return MG_OK;
+
+extern "C" {
+
+int mg_dual_fv_build(mg_ctx *c) {
+  API_BEGIN(c)
+  build_dual(c);
+  API_END(c)
+}
+
+int mg_dual_fv_get(mg_ctx *c, double *edge_surf, double *edge_n, double *cell_volume,
+                   double *sigma_el) {
+  API_BEGIN(c)
+  MG_CHECK(c->dual_ok, MG_ERR_STATE, "dual data not built");
+  if (edge_surf)
+    MG_HIP(hipMemcpyAsync(edge_surf, c->edge_surf, sizeof(double) * c->nE, hipMemcpyDeviceToHost,
+                          c->stream));
+  if (edge_n)
+    MG_HIP(hipMemcpyAsync(edge_n, c->edge_n, sizeof(double) * 3 * c->nE, hipMemcpyDeviceToHost,
+                          c->stream));
+  if (cell_volume)
+    MG_HIP(hipMemcpyAsync(cell_volume, c->cell_volume, sizeof(double) * c->nV,
+                          hipMemcpyDeviceToHost, c->stream));
+  if (sigma_el)
+    MG_HIP(hipMemcpyAsync(sigma_el, c->sigma_el, sizeof(double) * 6 * c->nT,
+                          hipMemcpyDeviceToHost, c->stream));
+  MG_HIP(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+
+int mg_dd_residual(mg_ctx *c, const double *eps, const double *rho, const double *mu,
+                   const double *phi, const double *dens, const uint8_t *fixed, double *r_phi,
+                   double *r_n, double *loss) {
+  API_BEGIN(c)
+  MG_CHECK(eps && rho && mu && phi && dens && fixed, MG_ERR_ARG, "dd: null input");
+  MG_CHECK(c->dual_ok, MG_ERR_STATE, "dd: dual data not built");
+  DDBuffers B = dd_upload(c, eps, rho, mu, phi, dens, fixed);
+  dd_residual_launch(c, B);
+  if (r_phi)
+    MG_HIP(hipMemcpyAsync(r_phi, B.rphi, sizeof(double) * c->nV, hipMemcpyDeviceToHost, c->stream));
+  if (r_n)
+    MG_HIP(hipMemcpyAsync(r_n, B.rn, sizeof(double) * c->nV, hipMemcpyDeviceToHost, c->stream));
+  if (loss) {
+    k_loss<<<1024, 256, 0, c->stream>>>(c->nV, B.rphi, B.rn, B.partial);
+    MG_HIP(hipGetLastError());
+    std::vector<double> part(1024);
+    MG_HIP(hipMemcpyAsync(part.data(), B.partial, sizeof(double) * 1024, hipMemcpyDeviceToHost,
+                          c->stream));
+    MG_HIP(hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (double v : part) s += v;
+    *loss = s;
+  }
+  MG_HIP(hipStreamSynchronize(c->stream));
+  dd_free(c, B);
+  API_END(c)
+}
+
+int mg_dd_vjp(mg_ctx *c, const double *eps, const double *rho, const double *mu,
+              const double *phi, const double *dens, const uint8_t *fixed, double *g_eps,
+              double *g_mu) {
+  API_BEGIN(c)
+  MG_CHECK(eps && rho && mu && phi && dens && fixed && g_eps && g_mu, MG_ERR_ARG,
+           "dd: null argument");
+  MG_CHECK(c->dual_ok, MG_ERR_STATE, "dd: dual data not built");
+  DDBuffers B = dd_upload(c, eps, rho, mu, phi, dens, fixed);
+  dd_residual_launch(c, B);
+  k_dd_vjp<<<blocks_for(c->nT, 128), 128, 0, c->stream>>>(c->nT, c->tets, c->xyz, c->sigma_el,
+                                                          B.phi, B.dens, B.rphi, B.rn, B.ge, B.gm);
+  MG_HIP(hipGetLastError());
+  MG_HIP(hipMemcpyAsync(g_eps, B.ge, sizeof(double) * c->nT, hipMemcpyDeviceToHost, c->stream));
+  MG_HIP(hipMemcpyAsync(g_mu, B.gm, sizeof(double) * c->nT, hipMemcpyDeviceToHost, c->stream));
+  MG_HIP(hipStreamSynchronize(c->stream));
+  dd_free(c, B);
+  API_END(c)
+}
+
+int mg_bernoulli(mg_ctx *c, int64_t n, const double *x, double *y) {
+  API_BEGIN(c)
+  MG_CHECK(n >= 0 && (n == 0 || (x && y)), MG_ERR_ARG, "bernoulli: bad arguments");
+  if (n == 0) return MG_OK;
+  double *dx = dalloc<double>(c, n), *dy = dalloc<double>(c, n);
+  MG_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  k_bernoulli<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, dx, dy);
+  MG_HIP(hipGetLastError());
+  MG_HIP(hipMemcpyAsync(y, dy, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  MG_HIP(hipStreamSynchronize(c->stream));
+  dfree(c, dx);
+  dfree(c, dy);
+  API_END(c)
+}
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+// Internal definitions shared by the translation units of libmgfe.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/mgfe.h"
+
+namespace mg {
+
+// One contiguous range of 64-row blocks launched with one LDS size.
+struct RowSegment {
+  int32_t row_begin;
+  int32_t row_end;
+  int32_t lds_slots;  // max CSR slots any 64-row block of the range owns
+};
+
+// Incidence record: element e sees row dof as its local basis l; pos[m] is
+// the position, inside that row, of the column dof of local basis m (0xFF when
+// the column is eliminated).  One 16-byte load for P2, 8 bytes for P1.
+struct alignas(16) RecP2 {
+  int32_t el;  // e << 4 | l
+  uint8_t pos[10];
+  uint8_t pad[2];
+};
+struct alignas(8) RecP1 {
+  int32_t el;
+  uint8_t pos[4];
+};
+
+struct System {
+  bool built = false;
+  int64_t nrows = 0, nnz = 0, n_inc = 0;
+  int max_row_nnz = 0;
+  int32_t *rowptr = nullptr;   // nrows + 1
+  int32_t *colidx = nullptr;   // nnz
+  int32_t *diagpos = nullptr;  // nrows, CSR index of the diagonal entry
+  int32_t *slot = nullptr;     // nT * nloc * nloc element -> CSR slot map
+  int32_t *inc_ptr = nullptr;  // nrows + 1
+  void *inc_rec = nullptr;     // n_inc records (RecP1 or RecP2)
+  std::vector<RowSegment> segs;
+  double *vals64 = nullptr;
+  float *vals32 = nullptr;
+  bool has64 = false, has32 = false;
+  double *dinv64 = nullptr;
+  float *dinv32 = nullptr;
+  bool dinv64_ok = false, dinv32_ok = false;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<void *> owned;  // every device allocation, freed by destroy
+
+  // mesh
+  int64_t nV = 0, nT = 0, nB = 0;
+  double *xyz = nullptr;   // 3 nV
+  int32_t *tets = nullptr; // 4 nT, 0-based
+  int32_t *btri = nullptr; // 3 nB, 0-based
+  int32_t *bgeo = nullptr; // nB
+  double *field[2] = {nullptr, nullptr};
+
+  // dof map
+  int order = 0, nloc = 0;
+  int64_t nE = 0, ndofs = 0;
+  uint64_t *edge_keys = nullptr;  // nE sorted (lo << 32 | hi)
+  int32_t *edge_nodes = nullptr;  // 2 nE
+  int32_t *elem_edge = nullptr;   // 6 nT
+  int32_t *dofs = nullptr;        // nloc nT
+  bool edges_ok = false;
+
+  // Dirichlet
+  uint8_t *fixed = nullptr;   // ndofs
+  int32_t *sysmap = nullptr;  // ndofs -> interior index or -1
+  int64_t nfree = 0;
+  bool dirichlet_ok = false;
+
+  System sys[2];
+
+  // per-element geometry/coefficient records for the current assembly
+  void *geom = nullptr;
+  size_t geom_bytes = 0;
+
+  // solver workspace
+  void *solve_ws = nullptr;
+  size_t solve_ws_bytes = 0;
+
+  // dual mesh
+  bool dual_ok = false;
+  double *edge_surf = nullptr, *edge_n = nullptr, *cell_volume = nullptr, *sigma_el = nullptr;
+  int32_t *vinc_ptr = nullptr;  // vertex -> (element, local vertex) incidence
+  int32_t *vinc = nullptr;
+
+  // scratch for library primitives (sorts, scans)
+  void *tmp = nullptr;
+  size_t tmp_bytes = 0;
+};
+
+// ---- error helpers ----------------------------------------------------------
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define MG_HIP(call)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      throw ::mg::Error{MG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+#define MG_CHECK(cond, code, msg)                  \
+  do {                                             \
+    if (!(cond)) throw ::mg::Error{(code), (msg)}; \
+  } while (0)
+
+template <typename T>
+T *dalloc(Ctx *c, size_t n) {
+  void *p = nullptr;
+  if (n == 0) n = 1;
+  MG_HIP(hipMalloc(&p, n * sizeof(T)));
+  c->owned.push_back(p);
+  return static_cast<T *>(p);
+}
+void dfree(Ctx *c, void *p);
+void *scratch(Ctx *c, size_t bytes);  // context-owned reusable temporary
+
+inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }
+
+// ---- cross-unit entry points ------------------------------------------------
+void build_edges(Ctx *c);
+void build_dofs(Ctx *c);
+void build_dirichlet(Ctx *c, int n_geo, const int32_t *geo_ids);
+void build_pattern(Ctx *c, int sys);
+void ensure_dinv(Ctx *c, int sys, int prec);
+void spmv_launch(Ctx *c, int sys, int prec, const void *x, void *y);
+
+// host-side reference tables (restating the reference basis and quadrature)
+int quad_rule(int npts, double *pts, double *w);
+void ref_basis(int order, const double *y, double *val, double *grad /* [3][n] */);
+// stiffness table: tab[k][l][m], k over (00,11,22,01,02,12); mass table tab[6][l][m]
+void build_tables(int order, int stiff_rule, int mass_rule, double *tab /* 7*100 */);
+
+}  // namespace mg
+
+struct mg_ctx : public mg::Ctx {};

@@ -1,0 +1,466 @@
+// Symbolic phase on the GPU: edge numbering, dof map, Dirichlet numbering,
+// CSR pattern, element -> CSR slot map and row-incidence records.
+//
+// Everything is sort-based (rocPRIM radix sort through hipCUB) so it scales to
+// the 10^8 (row, column) pairs of a 64^3 P2 mesh without host work, and every
+// result is deterministic (stable sorts, integer scans, no float atomics).
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+
+#include "mg_internal.h"
+
+namespace mg {
+
+__constant__ int c_edge_v[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+
+static int bitlen(uint64_t x) {
+  int b = 0;
+  while (x) {
+    ++b;
+    x >>= 1;
+  }
+  return b;
+}
+
+template <typename K, typename V>
+static void sort_pairs(Ctx *c, const K *kin, K *kout, const V *vin, V *vout, int64_t n,
+                       int end_bit) {
+  MG_CHECK(n < (1ll << 31), MG_ERR_SIZE, "sort: more than 2^31 items");
+  size_t tb = 0;
+  MG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kin, kout, vin, vout, (int)n, 0, end_bit,
+                                            c->stream));
+  void *t = scratch(c, tb);
+  MG_HIP(hipcub::DeviceRadixSort::SortPairs(t, tb, kin, kout, vin, vout, (int)n, 0, end_bit,
+                                            c->stream));
+}
+
+static void inclusive_sum(Ctx *c, const int32_t *in, int32_t *out, int64_t n) {
+  size_t tb = 0;
+  MG_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, in, out, (int)n, c->stream));
+  void *t = scratch(c, tb);
+  MG_HIP(hipcub::DeviceScan::InclusiveSum(t, tb, in, out, (int)n, c->stream));
+}
+
+static void exclusive_sum(Ctx *c, const int32_t *in, int32_t *out, int64_t n) {
+  size_t tb = 0;
+  MG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, (int)n, c->stream));
+  void *t = scratch(c, tb);
+  MG_HIP(hipcub::DeviceScan::ExclusiveSum(t, tb, in, out, (int)n, c->stream));
+}
+
+template <typename T>
+static T read_back(Ctx *c, const T *d) {
+  T h;
+  MG_HIP(hipMemcpyAsync(&h, d, sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  MG_HIP(hipStreamSynchronize(c->stream));
+  return h;
+}
+
+// ---- edges --------------------------------------------------------------------
+__global__ void k_edge_keys(int64_t nT, const int32_t *__restrict__ tets, uint64_t *__restrict__ keys,
+                            int32_t *__restrict__ vals) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 6 * nT) return;
+  int64_t e = i / 6;
+  int j = (int)(i - 6 * e);
+  uint32_t a = (uint32_t)tets[4 * e + c_edge_v[j][0]], b = (uint32_t)tets[4 * e + c_edge_v[j][1]];
+  uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+  keys[i] = ((uint64_t)lo << 32) | hi;
+  vals[i] = (int32_t)i;
+}
+
+__global__ void k_heads_u64(int64_t n, const uint64_t *__restrict__ keys, uint64_t invalid,
+                            int32_t *__restrict__ head) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t k = keys[i];
+  head[i] = (k != invalid) && (i == 0 || keys[i - 1] != k);
+}
+
+__global__ void k_edge_scatter(int64_t n, const uint64_t *__restrict__ keys,
+                               const int32_t *__restrict__ vals, const int32_t *__restrict__ id,
+                               const int32_t *__restrict__ head, int32_t *__restrict__ elem_edge,
+                               uint64_t *__restrict__ edge_keys, int32_t *__restrict__ edge_nodes) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t eid = id[i] - 1;
+  elem_edge[vals[i]] = eid;
+  if (head[i]) {
+    uint64_t k = keys[i];
+    edge_keys[eid] = k;
+    edge_nodes[2 * eid] = (int32_t)(k >> 32);
+    edge_nodes[2 * eid + 1] = (int32_t)(k & 0xffffffffu);
+  }
+}
+
+// connect3d::buildConnectivity (connect3d.hpp:417-491) + the single-rank
+// pMapItem global numbering (pVectGlobalManip.hpp:417-578): edge gid = rank
+// of the sorted (node, node) pair in lexicographic order.
+void build_edges(Ctx *c) {
+  const int64_t n = 6 * c->nT;
+  uint64_t *k0 = dalloc<uint64_t>(c, n), *k1 = dalloc<uint64_t>(c, n);
+  int32_t *v0 = dalloc<int32_t>(c, n), *v1 = dalloc<int32_t>(c, n);
+  int32_t *head = dalloc<int32_t>(c, n), *id = dalloc<int32_t>(c, n);
+  k_edge_keys<<<blocks_for(n, 256), 256, 0, c->stream>>>(c->nT, c->tets, k0, v0);
+  MG_HIP(hipGetLastError());
+  sort_pairs(c, k0, k1, v0, v1, n, 32 + bitlen((uint64_t)c->nV));
+  k_heads_u64<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, k1, ~0ull, head);
+  inclusive_sum(c, head, id, n);
+  c->nE = read_back(c, id + n - 1);
+  c->edge_keys = dalloc<uint64_t>(c, c->nE);
+  c->edge_nodes = dalloc<int32_t>(c, 2 * c->nE);
+  c->elem_edge = dalloc<int32_t>(c, n);
+  k_edge_scatter<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, k1, v1, id, head, c->elem_edge,
+                                                          c->edge_keys, c->edge_nodes);
+  MG_HIP(hipGetLastError());
+  MG_HIP(hipStreamSynchronize(c->stream));
+  for (void *p : {(void *)k0, (void *)k1, (void *)v0, (void *)v1, (void *)head, (void *)id})
+    dfree(c, p);
+  c->edges_ok = true;
+}
+
+// ---- dofs -----------------------------------------------------------------------
+// dofMapStatic3d::startup (dofMapStatic3d.hpp:471-527): vertex dofs first, then
+// one dof per edge at numVertices + edge gid.
+__global__ void k_dofs(int64_t nT, int nloc, int64_t nV, const int32_t *__restrict__ tets,
+                       const int32_t *__restrict__ elem_edge, int32_t *__restrict__ dofs) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nT) return;
+  for (int l = 0; l < 4; ++l) dofs[nloc * e + l] = tets[4 * e + l];
+  if (nloc == 10)
+    for (int j = 0; j < 6; ++j) dofs[10 * e + 4 + j] = (int32_t)(nV + elem_edge[6 * e + j]);
+}
+
+void build_dofs(Ctx *c) {
+  c->ndofs = c->order == 1 ? c->nV : c->nV + c->nE;
+  MG_CHECK(c->ndofs < (1ll << 31), MG_ERR_SIZE, "dofs: more than 2^31");
+  c->dofs = dalloc<int32_t>(c, c->nloc * c->nT);
+  k_dofs<<<blocks_for(c->nT, 256), 256, 0, c->stream>>>(c->nT, c->nloc, c->nV, c->tets,
+                                                        c->elem_edge, c->dofs);
+  MG_HIP(hipGetLastError());
+  MG_HIP(hipStreamSynchronize(c->stream));
+}
+
+// ---- Dirichlet ------------------------------------------------------------------
+__device__ int64_t find_edge(const uint64_t *__restrict__ keys, int64_t nE, uint64_t k) {
+  int64_t lo = 0, hi = nE - 1;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < k)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return (nE > 0 && keys[lo] == k) ? lo : -1;
+}
+
+__global__ void k_dirichlet(int64_t nB, const int32_t *__restrict__ btri,
+                            const int32_t *__restrict__ bgeo, int ngeo,
+                            const int32_t *__restrict__ geo, int order, int64_t nV, int64_t nE,
+                            const uint64_t *__restrict__ edge_keys, uint8_t *__restrict__ fixed,
+                            int *__restrict__ bad) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nB) return;
+  bool sel = false;
+  for (int g = 0; g < ngeo; ++g) sel |= bgeo[t] == geo[g];
+  if (!sel) return;
+  int v[3] = {btri[3 * t], btri[3 * t + 1], btri[3 * t + 2]};
+  for (int k = 0; k < 3; ++k) fixed[v[k]] = 1;
+  if (order == 2)
+    for (int k = 0; k < 3; ++k) {
+      uint32_t a = (uint32_t)v[k], b = (uint32_t)v[(k + 1) % 3];
+      uint64_t key = ((uint64_t)(a < b ? a : b) << 32) | (a < b ? b : a);
+      int64_t id = find_edge(edge_keys, nE, key);
+      if (id < 0)
+        atomicAdd(bad, 1);
+      else
+        fixed[nV + id] = 1;
+    }
+}
+
+__global__ void k_free_flags(int64_t n, const uint8_t *__restrict__ fixed, int32_t *__restrict__ f) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = fixed[i] ? 0 : 1;
+}
+
+__global__ void k_sysmap(int64_t n, const uint8_t *__restrict__ fixed,
+                         const int32_t *__restrict__ scan, int32_t *__restrict__ map) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) map[i] = fixed[i] ? -1 : scan[i];
+}
+
+void build_dirichlet(Ctx *c, int n_geo, const int32_t *geo_ids) {
+  const int64_t N = c->ndofs;
+  if (!c->fixed) c->fixed = dalloc<uint8_t>(c, N);
+  if (!c->sysmap) c->sysmap = dalloc<int32_t>(c, N);
+  int32_t *dgeo = dalloc<int32_t>(c, 64);
+  int *bad = dalloc<int>(c, 1);
+  MG_HIP(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+  MG_HIP(hipMemsetAsync(c->fixed, 0, N, c->stream));
+  if (n_geo)
+    MG_HIP(hipMemcpyAsync(dgeo, geo_ids, sizeof(int32_t) * n_geo, hipMemcpyHostToDevice,
+                          c->stream));
+  if (c->nB && n_geo)
+    k_dirichlet<<<blocks_for(c->nB, 256), 256, 0, c->stream>>>(c->nB, c->btri, c->bgeo, n_geo,
+                                                               dgeo, c->order, c->nV, c->nE,
+                                                               c->edge_keys, c->fixed, bad);
+  int32_t *f = dalloc<int32_t>(c, N + 1), *s = dalloc<int32_t>(c, N + 1);
+  k_free_flags<<<blocks_for(N, 256), 256, 0, c->stream>>>(N, c->fixed, f);
+  MG_HIP(hipMemsetAsync(f + N, 0, sizeof(int32_t), c->stream));
+  exclusive_sum(c, f, s, N + 1);
+  k_sysmap<<<blocks_for(N, 256), 256, 0, c->stream>>>(N, c->fixed, s, c->sysmap);
+  MG_HIP(hipGetLastError());
+  c->nfree = read_back(c, s + N);
+  int hb = read_back(c, bad);
+  for (void *p : {(void *)dgeo, (void *)bad, (void *)f, (void *)s}) dfree(c, p);
+  MG_CHECK(hb == 0, MG_ERR_MESH, "dirichlet: a boundary triangle edge is not a mesh edge");
+  // a new Dirichlet set invalidates the interior system
+  System &S = c->sys[MG_SYS_INTERIOR];
+  for (void *p : {(void *)S.rowptr, (void *)S.colidx, (void *)S.diagpos, (void *)S.slot,
+                  (void *)S.inc_ptr, S.inc_rec, (void *)S.vals64, (void *)S.vals32,
+                  (void *)S.dinv64, (void *)S.dinv32})
+    dfree(c, p);
+  S = System();
+  c->dirichlet_ok = true;
+}
+
+// ---- pattern ----------------------------------------------------------------------
+// Contribution c = e*nl*nl + i*nl + j is the entry (row = test basis j, column =
+// field basis i) of element e's column-major local matrix, the order in which
+// traitsMatrixBuilder::buildEpetraCrs hands it to InsertGlobalValues(COLUMN_MAJOR).
+__global__ void k_pattern_keys(int64_t nT, int nl, const int32_t *__restrict__ dofs,
+                               const int32_t *__restrict__ map, uint64_t nrows, uint64_t invalid,
+                               uint64_t *__restrict__ keys, int32_t *__restrict__ vals) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t n = nT * nl * nl;
+  if (c >= n) return;
+  int64_t e = c / (nl * nl);
+  int rem = (int)(c - e * nl * nl);
+  int i = rem / nl, j = rem - i * nl;
+  int32_t row = dofs[e * nl + j], col = dofs[e * nl + i];
+  if (map) {
+    row = map[row];
+    col = map[col];
+  }
+  keys[c] = (row < 0 || col < 0) ? invalid : (uint64_t)row * nrows + (uint64_t)col;
+  vals[c] = (int32_t)c;
+}
+
+__global__ void k_pattern_scatter(int64_t n, const uint64_t *__restrict__ keys,
+                                  const int32_t *__restrict__ vals, const int32_t *__restrict__ id,
+                                  const int32_t *__restrict__ head, uint64_t nrows,
+                                  uint64_t invalid, int32_t *__restrict__ slot,
+                                  int32_t *__restrict__ colidx, int32_t *__restrict__ rowcnt) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t k = keys[i];
+  if (k == invalid) {
+    slot[vals[i]] = -1;
+    return;
+  }
+  int32_t s = id[i] - 1;
+  slot[vals[i]] = s;
+  if (head[i]) {
+    colidx[s] = (int32_t)(k % nrows);
+    atomicAdd(&rowcnt[k / nrows], 1);
+  }
+}
+
+__global__ void k_diagpos(int64_t nrows, const int32_t *__restrict__ rowptr,
+                          const int32_t *__restrict__ colidx, int32_t *__restrict__ diag) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  int lo = rowptr[r], hi = rowptr[r + 1] - 1, d = -1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    int cc = colidx[mid];
+    if (cc == r) {
+      d = mid;
+      break;
+    }
+    if (cc < r)
+      lo = mid + 1;
+    else
+      hi = mid - 1;
+  }
+  diag[r] = d;
+}
+
+__global__ void k_inc_keys(int64_t nT, int nl, const int32_t *__restrict__ dofs,
+                           const int32_t *__restrict__ map, uint64_t invalid,
+                           uint64_t *__restrict__ keys, int32_t *__restrict__ vals) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nT * nl) return;
+  int64_t e = p / nl;
+  int l = (int)(p - e * nl);
+  int32_t row = dofs[p];
+  if (map) row = map[row];
+  keys[p] = row < 0 ? invalid : (uint64_t)row * nl + l;
+  vals[p] = (int32_t)((e << 4) | l);
+}
+
+__global__ void k_inc_count(int64_t n, const uint64_t *__restrict__ keys, uint64_t invalid, int nl,
+                            int32_t *__restrict__ cnt) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t k = keys[i];
+  if (k != invalid) atomicAdd(&cnt[k / (uint64_t)nl], 1);
+}
+
+template <int NL, typename Rec>
+__global__ void k_inc_records(int64_t n_inc, const uint64_t *__restrict__ keys,
+                              const int32_t *__restrict__ vals, const int32_t *__restrict__ slot,
+                              const int32_t *__restrict__ rowptr, Rec *__restrict__ rec,
+                              int *__restrict__ bad) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_inc) return;
+  int32_t v = vals[i];
+  int64_t e = v >> 4;
+  int l = v & 15;
+  int64_t row = (int64_t)(keys[i] / NL);
+  int32_t base = rowptr[row];
+  Rec r;
+  r.el = v;
+  for (int m = 0; m < NL; ++m) {
+    int32_t s = slot[e * NL * NL + m * NL + l];  // row basis l, column basis m
+    int p = s < 0 ? 0xFF : s - base;
+    if (s >= 0 && (p < 0 || p >= 0xFF)) {
+      atomicAdd(bad, 1);
+      p = 0xFF;
+    }
+    r.pos[m] = (uint8_t)p;
+  }
+  rec[i] = r;
+}
+
+static void make_segments(System &S, const std::vector<int32_t> &rp) {
+  S.segs.clear();
+  const int64_t nb = (S.nrows + 63) / 64;
+  int maxnnz = 0;
+  for (int64_t r = 0; r < S.nrows; ++r) maxnnz = std::max(maxnnz, rp[r + 1] - rp[r]);
+  S.max_row_nnz = maxnnz;
+  // split rows into runs whose 64-row blocks need a similar LDS size
+  auto need = [&](int64_t b) {
+    int64_t r0 = b * 64, r1 = std::min<int64_t>(r0 + 64, S.nrows);
+    return rp[r1] - rp[r0];
+  };
+  auto bucket = [](int x) {
+    int b = 0;
+    while ((1 << b) < x) ++b;
+    return b;
+  };
+  int64_t b = 0;
+  while (b < nb) {
+    int64_t start = b;
+    int bk = bucket(need(b));
+    int mx = need(b);
+    ++b;
+    while (b < nb && (bucket(need(b)) == bk || b - start < 256)) {
+      mx = std::max(mx, need(b));
+      ++b;
+    }
+    if (!S.segs.empty() && b - start < 256) {
+      S.segs.back().row_end = (int32_t)std::min<int64_t>(b * 64, S.nrows);
+      S.segs.back().lds_slots = std::max(S.segs.back().lds_slots, mx);
+    } else {
+      S.segs.push_back(RowSegment{(int32_t)(start * 64), (int32_t)std::min<int64_t>(b * 64, S.nrows),
+                                  mx});
+    }
+  }
+}
+
+void build_pattern(Ctx *c, int sys) {
+  System &S = c->sys[sys];
+  for (void *p : {(void *)S.rowptr, (void *)S.colidx, (void *)S.diagpos, (void *)S.slot,
+                  (void *)S.inc_ptr, S.inc_rec, (void *)S.vals64, (void *)S.vals32,
+                  (void *)S.dinv64, (void *)S.dinv32})
+    dfree(c, p);
+  S = System();
+  const int nl = c->nloc;
+  const int32_t *map = sys == MG_SYS_FULL ? nullptr : c->sysmap;
+  const int64_t nrows = sys == MG_SYS_FULL ? c->ndofs : c->nfree;
+  S.nrows = nrows;
+  MG_CHECK(nrows > 0, MG_ERR_STATE, "pattern: no rows");
+  const int64_t ncon = c->nT * nl * nl;
+  MG_CHECK(ncon < (1ll << 31), MG_ERR_SIZE, "pattern: more than 2^31 element contributions");
+  const int eb = bitlen((uint64_t)nrows * (uint64_t)nrows) + 1;
+  const uint64_t invalid = (eb >= 64) ? ~0ull : ((1ull << eb) - 1);
+
+  uint64_t *k0 = dalloc<uint64_t>(c, ncon), *k1 = dalloc<uint64_t>(c, ncon);
+  int32_t *v0 = dalloc<int32_t>(c, ncon), *v1 = dalloc<int32_t>(c, ncon);
+  k_pattern_keys<<<blocks_for(ncon, 256), 256, 0, c->stream>>>(c->nT, nl, c->dofs, map,
+                                                               (uint64_t)nrows, invalid, k0, v0);
+  MG_HIP(hipGetLastError());
+  sort_pairs(c, k0, k1, v0, v1, ncon, eb);
+  dfree(c, k0);
+  dfree(c, v0);
+  int32_t *head = dalloc<int32_t>(c, ncon), *id = dalloc<int32_t>(c, ncon);
+  k_heads_u64<<<blocks_for(ncon, 256), 256, 0, c->stream>>>(ncon, k1, invalid, head);
+  inclusive_sum(c, head, id, ncon);
+  S.nnz = read_back(c, id + ncon - 1);
+  S.slot = dalloc<int32_t>(c, ncon);
+  S.colidx = dalloc<int32_t>(c, S.nnz);
+  int32_t *cnt = dalloc<int32_t>(c, nrows + 1);
+  S.rowptr = dalloc<int32_t>(c, nrows + 1);
+  MG_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (nrows + 1), c->stream));
+  k_pattern_scatter<<<blocks_for(ncon, 256), 256, 0, c->stream>>>(
+      ncon, k1, v1, id, head, (uint64_t)nrows, invalid, S.slot, S.colidx, cnt);
+  MG_HIP(hipGetLastError());
+  exclusive_sum(c, cnt, S.rowptr, nrows + 1);
+  S.diagpos = dalloc<int32_t>(c, nrows);
+  k_diagpos<<<blocks_for(nrows, 256), 256, 0, c->stream>>>(nrows, S.rowptr, S.colidx, S.diagpos);
+  MG_HIP(hipGetLastError());
+  for (void *p : {(void *)k1, (void *)v1, (void *)head, (void *)id}) dfree(c, p);
+
+  // row incidence: (element, local basis) pairs grouped by row, ordered by
+  // (local basis, element) inside a row
+  const int64_t npair = c->nT * nl;
+  const int eb2 = bitlen((uint64_t)nrows * nl) + 1;
+  const uint64_t inv2 = (1ull << eb2) - 1;
+  k0 = dalloc<uint64_t>(c, npair);
+  k1 = dalloc<uint64_t>(c, npair);
+  v0 = dalloc<int32_t>(c, npair);
+  v1 = dalloc<int32_t>(c, npair);
+  k_inc_keys<<<blocks_for(npair, 256), 256, 0, c->stream>>>(c->nT, nl, c->dofs, map, inv2, k0, v0);
+  MG_HIP(hipGetLastError());
+  sort_pairs(c, k0, k1, v0, v1, npair, eb2);
+  MG_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (nrows + 1), c->stream));
+  k_inc_count<<<blocks_for(npair, 256), 256, 0, c->stream>>>(npair, k1, inv2, nl, cnt);
+  S.inc_ptr = dalloc<int32_t>(c, nrows + 1);
+  exclusive_sum(c, cnt, S.inc_ptr, nrows + 1);
+  S.n_inc = read_back(c, S.inc_ptr + nrows);
+  int *bad = dalloc<int>(c, 1);
+  MG_HIP(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+  if (nl == 10) {
+    S.inc_rec = dalloc<RecP2>(c, S.n_inc);
+    k_inc_records<10, RecP2><<<blocks_for(S.n_inc, 256), 256, 0, c->stream>>>(
+        S.n_inc, k1, v1, S.slot, S.rowptr, (RecP2 *)S.inc_rec, bad);
+  } else {
+    S.inc_rec = dalloc<RecP1>(c, S.n_inc);
+    k_inc_records<4, RecP1><<<blocks_for(S.n_inc, 256), 256, 0, c->stream>>>(
+        S.n_inc, k1, v1, S.slot, S.rowptr, (RecP1 *)S.inc_rec, bad);
+  }
+  MG_HIP(hipGetLastError());
+  int hb = read_back(c, bad);
+  for (void *p : {(void *)k0, (void *)k1, (void *)v0, (void *)v1, (void *)cnt, (void *)bad})
+    dfree(c, p);
+  MG_CHECK(hb == 0, MG_ERR_SIZE, "pattern: a row has more than 254 entries (uint8 positions)");
+
+  std::vector<int32_t> rp(nrows + 1);
+  MG_HIP(hipMemcpyAsync(rp.data(), S.rowptr, sizeof(int32_t) * (nrows + 1), hipMemcpyDeviceToHost,
+                        c->stream));
+  MG_HIP(hipStreamSynchronize(c->stream));
+  make_segments(S, rp);
+  for (auto &sg : S.segs)
+    MG_CHECK(sg.lds_slots * 8 <= 64 * 1024, MG_ERR_SIZE,
+             "pattern: a 64-row block owns more than 8192 CSR entries");
+  S.vals64 = dalloc<double>(c, S.nnz);
+  S.vals32 = dalloc<float>(c, S.nnz);
+  S.dinv64 = dalloc<double>(c, nrows);
+  S.dinv32 = dalloc<float>(c, nrows);
+  S.built = true;
+}
+
+}  // namespace mg

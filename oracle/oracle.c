@@ -1,0 +1,835 @@
+/*
+ * ORACLE — test infrastructure only (see oracle.h).  FP64 CPU restatement of
+ * Morgana's FE assembly + Jacobi-PCG path, plus the dual-mesh data the
+ * Scharfetter-Gummel residual uses.  Never linked into libmgfe.so.
+ *
+ * Pins (independent checks of this file, in tests/test_oracle_*.py):
+ *   quadrature    monomial exactness a!b!c!/(a+b+c+3)! (deg 1/2/4/5)
+ *   basis         golden logs fePr3dTest2.log / fePr3dTest3.log, partition of
+ *                 unity, Kronecker property at the reference nodes
+ *   geometry      geoMapInterfaceTest5.log, P2 global grads on the reflected tet
+ *   element mats  P1 closed forms (K = c vol G G^T, M = vol/20 (1+delta)),
+ *                 exact rational P2 integrals, symmetry, K*1 = 0, sum(M) = vol
+ *   numbering     nnz closed forms for the Kuhn cube, brute-force pairs
+ *   assembly      scipy.sparse COO->CSR of the same element matrices
+ *   PCG           scipy.sparse.linalg.spsolve on the same system
+ *   dual FV data  sum(cellVolume) = |cube|, cellVolume(v) = sum vol(T)/4,
+ *                 per-element split sums to edgeSurf
+ *   SG residual   Bernoulli identities, linear-profile closed form
+ *   gradients     central finite differences of the oracle's own loss
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------- */
+/* Quadrature on the unit tetrahedron (weights sum to 1/6).                   */
+/* npts = 1, 4, 15 restate intStaticCards.cpp:705-829                         */
+/*   (intStaticCard<linearTetra,STANDARD,1|2|3>).                             */
+/* npts = 11 is Keast's degree-4 rule (not in the reference; the build's      */
+/*   choice for P2).  Pinned by monomial exactness in the tests.              */
+/* Returns the number of points, 0 for an unknown rule.                       */
+/* ------------------------------------------------------------------------- */
+int or_quad_rule(int npts, double *pts, double *w) {
+  int q = 0;
+#define PUT(X, Y, Z, W) \
+  do {                  \
+    pts[3 * q] = (X);   \
+    pts[3 * q + 1] = (Y); \
+    pts[3 * q + 2] = (Z); \
+    w[q] = (W);         \
+    ++q;                \
+  } while (0)
+  if (npts == 1) { /* intStaticCards.cpp:705-727 */
+    PUT(0.25, 0.25, 0.25, 1.0 / 6.0);
+  } else if (npts == 4) { /* intStaticCards.cpp:731-760 */
+    const double a = (5.0 - sqrt(5.0)) / 20.0;
+    const double b = (5.0 + 3 * sqrt(5.0)) / 20.0;
+    PUT(a, a, a, 1.0 / 24.0);
+    PUT(a, a, b, 1.0 / 24.0);
+    PUT(a, b, a, 1.0 / 24.0);
+    PUT(b, a, a, 1.0 / 24.0);
+  } else if (npts == 15) { /* intStaticCards.cpp:764-829 (constants as printed there) */
+    const double r5 = 0.25;
+    const double s5[2] = {0.09197107805272303, 0.3197936278296299};
+    const double t5[2] = {0.7240867658418310, 0.04061911651111023};
+    const double u5 = 0.05635083268962915, v5 = 0.4436491673103708;
+    const double A5 = 0.01975308641975309;
+    const double B5[2] = {0.01198951396316977, 0.01151136787104540};
+    const double C5 = 0.008818342151675485;
+    PUT(r5, r5, r5, A5);
+    for (int k = 0; k < 2; ++k) {
+      PUT(s5[k], s5[k], s5[k], B5[k]);
+      PUT(t5[k], s5[k], s5[k], B5[k]);
+      PUT(s5[k], t5[k], s5[k], B5[k]);
+      PUT(s5[k], s5[k], t5[k], B5[k]);
+    }
+    PUT(u5, u5, v5, C5);
+    PUT(u5, v5, u5, C5);
+    PUT(v5, u5, u5, C5);
+    PUT(v5, v5, u5, C5);
+    PUT(v5, u5, v5, C5);
+    PUT(u5, v5, v5, C5);
+  } else if (npts == 11) { /* Keast (1986), degree 4, one negative weight */
+    const double w0 = -74.0 / 5625.0, w1 = 343.0 / 45000.0, w2 = 56.0 / 2250.0;
+    const double c1 = 1.0 / 14.0, d1 = 11.0 / 14.0;
+    const double a = (1.0 + sqrt(5.0 / 14.0)) / 4.0, b = (1.0 - sqrt(5.0 / 14.0)) / 4.0;
+    PUT(0.25, 0.25, 0.25, w0);
+    PUT(c1, c1, c1, w1);
+    PUT(d1, c1, c1, w1);
+    PUT(c1, d1, c1, w1);
+    PUT(c1, c1, d1, w1);
+    PUT(a, b, b, w2);
+    PUT(b, a, b, w2);
+    PUT(b, b, a, w2);
+    PUT(a, a, b, w2);
+    PUT(a, b, a, w2);
+    PUT(b, a, a, w2);
+  }
+#undef PUT
+  return q;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Reference basis as monomial sums sum_k C_k x^px y^py z^pz, evaluated the   */
+/* way polyStatic does (polyStatic.hpp:89-144), with the monomial tables of   */
+/* polyCards.h:349-429 / polyCards.cpp:233-285 and the label order of         */
+/* feStaticLists.hpp:141-157 (4 vertices, then edges 12,23,31,14,24,34).      */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  int n;
+  double c[10];
+  int p[10][3];
+} or_poly;
+
+static const or_poly P1_POLYS[4] = {
+    {4, {1.0, -1.0, -1.0, -1.0}, {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}}},
+    {1, {1.0}, {{1, 0, 0}}},
+    {1, {1.0}, {{0, 1, 0}}},
+    {1, {1.0}, {{0, 0, 1}}},
+};
+
+static const or_poly P2_POLYS[10] = {
+    {10,
+     {1.0, -3.0, -3.0, -3.0, 2.0, 2.0, 2.0, 4.0, 4.0, 4.0},
+     {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0},
+      {1, 0, 1}, {0, 1, 1}}},
+    {2, {2.0, -1.0}, {{2, 0, 0}, {1, 0, 0}}},
+    {2, {2.0, -1.0}, {{0, 2, 0}, {0, 1, 0}}},
+    {2, {2.0, -1.0}, {{0, 0, 2}, {0, 0, 1}}},
+    {4, {4.0, -4.0, -4.0, -4.0}, {{1, 0, 0}, {2, 0, 0}, {1, 1, 0}, {1, 0, 1}}},
+    {1, {4.0}, {{1, 1, 0}}},
+    {4, {4.0, -4.0, -4.0, -4.0}, {{0, 1, 0}, {1, 1, 0}, {0, 2, 0}, {0, 1, 1}}},
+    {4, {4.0, -4.0, -4.0, -4.0}, {{0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}}},
+    {1, {4.0}, {{1, 0, 1}}},
+    {1, {4.0}, {{0, 1, 1}}},
+};
+
+static double ipow(double x, int p) {
+  double r = 1.0;
+  for (int k = 0; k < p; ++k) r *= x;
+  return r;
+}
+
+/* d^(dx,dy,dz) of one monomial, as polyStatic's derivative recursion */
+static double mono(const or_poly *P, const double *y, int d) {
+  double s = 0.0;
+  for (int k = 0; k < P->n; ++k) {
+    int px = P->p[k][0], py = P->p[k][1], pz = P->p[k][2];
+    double c = P->c[k];
+    if (d == 0) {
+      s += c * ipow(y[0], px) * ipow(y[1], py) * ipow(y[2], pz);
+    } else if (d == 1) {
+      if (px > 0) s += c * px * ipow(y[0], px - 1) * ipow(y[1], py) * ipow(y[2], pz);
+    } else if (d == 2) {
+      if (py > 0) s += c * py * ipow(y[0], px) * ipow(y[1], py - 1) * ipow(y[2], pz);
+    } else {
+      if (pz > 0) s += c * pz * ipow(y[0], px) * ipow(y[1], py) * ipow(y[2], pz - 1);
+    }
+  }
+  return s;
+}
+
+/* fePr3d<1|2>::localEval / localGrad (fePr3d.hpp:245-451, 453-641) */
+void or_basis(int order, const double *y, double *val, double *gx, double *gy, double *gz) {
+  const or_poly *T = order == 1 ? P1_POLYS : P2_POLYS;
+  int n = order == 1 ? 4 : 10;
+  for (int i = 0; i < n; ++i) {
+    if (val) val[i] = mono(&T[i], y, 0);
+    if (gx) gx[i] = mono(&T[i], y, 1);
+    if (gy) gy[i] = mono(&T[i], y, 2);
+    if (gz) gz[i] = mono(&T[i], y, 3);
+  }
+}
+
+/* geoMapInterface<linearTetra>::getGradient (geoMapInterface.hpp:374-382):   */
+/* columns dx/dY = P2-P1, P3-P1, P4-P1 placed by the tensor3d columns ctor    */
+/* (tensor3d.cpp:41-54).  J is row-major, J[3*r+c] = d x_r / d Y_c.           */
+void or_tet_jacobian(const double *P, double *J) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) J[3 * r + c] = P[3 * (c + 1) + r] - P[r];
+}
+
+/* tensor3d::getThirdInvariant (tensor3d.cpp:478-484), signed */
+double or_det3(const double *T) {
+#define E(i, j) T[3 * (i) + (j)]
+  return E(0, 0) * E(1, 1) * E(2, 2) - E(0, 0) * E(1, 2) * E(2, 1) - E(1, 0) * E(0, 1) * E(2, 2) +
+         E(1, 0) * E(0, 2) * E(2, 1) + E(2, 0) * E(0, 1) * E(1, 2) - E(2, 0) * E(0, 2) * E(1, 1);
+}
+
+/* tensor3d::computeInverse (tensor3d.cpp:400-430), cofactors / det */
+void or_inv3(const double *T, double *I) {
+  double d = or_det3(T);
+  I[0] = (E(1, 1) * E(2, 2) - E(1, 2) * E(2, 1)) / d;
+  I[1] = -(E(0, 1) * E(2, 2) - E(0, 2) * E(2, 1)) / d;
+  I[2] = (E(0, 1) * E(1, 2) - E(0, 2) * E(1, 1)) / d;
+  I[3] = -(E(1, 0) * E(2, 2) - E(1, 2) * E(2, 0)) / d;
+  I[4] = (E(0, 0) * E(2, 2) - E(0, 2) * E(2, 0)) / d;
+  I[5] = -(E(0, 0) * E(1, 2) - E(0, 2) * E(1, 0)) / d;
+  I[6] = (E(1, 0) * E(2, 1) - E(1, 1) * E(2, 0)) / d;
+  I[7] = -(E(0, 0) * E(2, 1) - E(0, 1) * E(2, 0)) / d;
+  I[8] = (E(0, 0) * E(1, 1) - E(0, 1) * E(1, 0)) / d;
+#undef E
+}
+
+/* fePr3d::globalGrad (fePr3d.hpp:421-451 P1, 643-667 P2):                    */
+/* grad_x(i) = locX(i) T(1,1) + locY(i) T(2,1) + locZ(i) T(3,1), T = J^{-1}   */
+void or_global_grad(int order, const double *P, const double *y, double *gx, double *gy,
+                    double *gz) {
+  double J[9], T[9], lx[10], ly[10], lz[10];
+  int n = order == 1 ? 4 : 10;
+  or_tet_jacobian(P, J);
+  or_inv3(J, T);
+  or_basis(order, y, NULL, lx, ly, lz);
+  for (int i = 0; i < n; ++i) {
+    gx[i] = lx[i] * T[0] + ly[i] * T[3] + lz[i] * T[6];
+    gy[i] = lx[i] * T[1] + ly[i] * T[4] + lz[i] * T[7];
+    gz[i] = lx[i] * T[2] + ly[i] * T[5] + lz[i] * T[8];
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Element matrix, column-major as the reference builds it:                   */
+/*   mat[i*n + j], i = field (column) basis, j = test (row) basis             */
+/* restating integratorOEL3d_STD::integration (integratorOEL3d_STD.hpp:142-170)*/
+/* around opEL3dA::eval (opEL3dA.hpp:129-165; opEL3dF with constant c) and     */
+/* opEL3dG/opEL3dL::eval (opEL3dG.hpp:83-108; opEL3dL.hpp:129).  Geometry is  */
+/* recomputed per quadrature point as the reference does.  det is signed.     */
+/* ------------------------------------------------------------------------- */
+int or_element_matrix(int ops, int order, const double *P, double c_stiff, double c_mass,
+                      int rule_stiff, int rule_mass, double *mat) {
+  int n = order == 1 ? 4 : 10;
+  double pts[3 * 15], w[15], buf[100];
+  memset(mat, 0, sizeof(double) * n * n);
+  if (ops & OR_OP_STIFF) {
+    int nq = or_quad_rule(rule_stiff, pts, w);
+    if (!nq) return -1;
+    for (int q = 0; q < nq; ++q) {
+      double gx[10], gy[10], gz[10], J[9];
+      or_global_grad(order, P, &pts[3 * q], gx, gy, gz);
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+          buf[i * n + j] = (gx[i] * gx[j] + gy[i] * gy[j] + gz[i] * gz[j]) * c_stiff;
+      or_tet_jacobian(P, J);
+      double jac = or_det3(J);
+      for (int k = 0; k < n * n; ++k) mat[k] += buf[k] * jac * w[q];
+    }
+  }
+  if (ops & OR_OP_MASS) {
+    int nq = or_quad_rule(rule_mass, pts, w);
+    if (!nq) return -1;
+    double acc[100];
+    memset(acc, 0, sizeof(acc));
+    for (int q = 0; q < nq; ++q) {
+      double v[10], J[9];
+      or_basis(order, &pts[3 * q], v, NULL, NULL, NULL);
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) buf[i * n + j] = v[i] * v[j] * c_mass;
+      or_tet_jacobian(P, J);
+      double jac = or_det3(J);
+      for (int k = 0; k < n * n; ++k) acc[k] += buf[k] * jac * w[q];
+    }
+    /* the two operators are assembled separately by the reference and added   */
+    /* (epetraMatrixManip::sum, epetraMatrixManip.cpp:510-527)                 */
+    for (int k = 0; k < n * n; ++k) mat[k] += acc[k];
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Edge enumeration and numbering.                                            */
+/* connect3d::buildConnectivity (connect3d.hpp:417-491) walks local edges     */
+/* with linearTetra::edgeToPoint {1,2, 2,3, 3,1, 1,4, 2,4, 3,4}               */
+/* (geoShapes.cpp:670-677); on one rank with pMapItem maps the global edge    */
+/* gid is the lexicographic rank of the sorted node pair                      */
+/* (pVectGlobalManip.hpp:417-578, pGraphItem.cpp:238-260).                    */
+/* Output: elem_edge[6T] 0-based edge ids, edge_nodes[2E] sorted 0-based.     */
+/* ------------------------------------------------------------------------- */
+static const int EDGE_V[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+
+static int cmp_u64(const void *a, const void *b) {
+  uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+static uint64_t edge_key(const int32_t *tets, int64_t e, int j) {
+  uint64_t a = (uint64_t)(tets[4 * e + EDGE_V[j][0]] - 1);
+  uint64_t b = (uint64_t)(tets[4 * e + EDGE_V[j][1]] - 1);
+  if (a > b) {
+    uint64_t t = a;
+    a = b;
+    b = t;
+  }
+  return (a << 32) | b;
+}
+
+int64_t or_edges(int64_t nV, int64_t nT, const int32_t *tets, int32_t *elem_edge,
+                 int32_t *edge_nodes) {
+  (void)nV;
+  uint64_t *k = (uint64_t *)malloc(sizeof(uint64_t) * 6 * (size_t)nT);
+  for (int64_t e = 0; e < nT; ++e)
+    for (int j = 0; j < 6; ++j) k[6 * e + j] = edge_key(tets, e, j);
+  qsort(k, 6 * (size_t)nT, sizeof(uint64_t), cmp_u64);
+  int64_t E = 0;
+  for (int64_t i = 0; i < 6 * nT; ++i)
+    if (i == 0 || k[i] != k[i - 1]) k[E++] = k[i];
+  for (int64_t e = 0; e < nT; ++e)
+    for (int j = 0; j < 6; ++j) {
+      uint64_t key = edge_key(tets, e, j);
+      int64_t lo = 0, hi = E - 1;
+      while (lo < hi) {
+        int64_t mid = (lo + hi) / 2;
+        if (k[mid] < key)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      elem_edge[6 * e + j] = (int32_t)lo;
+    }
+  if (edge_nodes)
+    for (int64_t i = 0; i < E; ++i) {
+      edge_nodes[2 * i] = (int32_t)(k[i] >> 32);
+      edge_nodes[2 * i + 1] = (int32_t)(k[i] & 0xffffffffu);
+    }
+  free(k);
+  return E;
+}
+
+/* dofMapStatic3d::startup (dofMapStatic3d.hpp:471-527) with                  */
+/* feStaticInterface3d::feIndicesLG (feStaticInterface3d.hpp:389-408):        */
+/* vertex dof gid = node gid, edge dof gid = numVerticesG + edge gid.         */
+/* 0-based output dofs[T * nloc].                                             */
+void or_elem_dofs(int order, int64_t nV, int64_t nT, const int32_t *tets,
+                  const int32_t *elem_edge, int32_t *dofs) {
+  int n = order == 1 ? 4 : 10;
+  for (int64_t e = 0; e < nT; ++e) {
+    for (int l = 0; l < 4; ++l) dofs[n * e + l] = tets[4 * e + l] - 1;
+    if (order == 2)
+      for (int j = 0; j < 6; ++j) dofs[n * e + 4 + j] = (int32_t)(nV + elem_edge[6 * e + j]);
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Global assembly: element loop + InsertGlobalValues(COLUMN_MAJOR) +         */
+/* GlobalAssemble sort/merge (traitsMatrixBuilder.hpp:366-441).  Duplicate    */
+/* (row, col) pairs are summed in insertion (element) order; every structural */
+/* pair is kept, zeros included.                                              */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  uint64_t key;
+  uint64_t idx;
+} or_kv;
+
+static int cmp_kv(const void *a, const void *b) {
+  const or_kv *x = (const or_kv *)a, *y = (const or_kv *)b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  return x->idx < y->idx ? -1 : (x->idx > y->idx ? 1 : 0);
+}
+
+int or_assemble(int order, int ops, int64_t nV, const double *xyz, int64_t nT,
+                const int32_t *tets, const double *c_stiff, const double *c_mass,
+                int rule_stiff, int rule_mass, int64_t *nrows, int64_t *nnz,
+                int32_t **rowptr, int32_t **colidx, double **vals) {
+  int n = order == 1 ? 4 : 10;
+  int32_t *ee = NULL;
+  int64_t N = nV;
+  if (order == 2) {
+    ee = (int32_t *)malloc(sizeof(int32_t) * 6 * (size_t)nT);
+    N = nV + or_edges(nV, nT, tets, ee, NULL);
+  }
+  int32_t *dofs = (int32_t *)malloc(sizeof(int32_t) * n * (size_t)nT);
+  or_elem_dofs(order, nV, nT, tets, ee, dofs);
+  size_t ncoo = (size_t)nT * n * n;
+  or_kv *kv = (or_kv *)malloc(sizeof(or_kv) * ncoo);
+  double *cv = (double *)malloc(sizeof(double) * ncoo);
+  double P[12], mat[100];
+  for (int64_t e = 0; e < nT; ++e) {
+    for (int v = 0; v < 4; ++v)
+      for (int d = 0; d < 3; ++d) P[3 * v + d] = xyz[3 * (tets[4 * e + v] - 1) + d];
+    double cs = c_stiff ? c_stiff[e] : 1.0, cm = c_mass ? c_mass[e] : 1.0;
+    if (or_element_matrix(ops, order, P, cs, cm, rule_stiff, rule_mass, mat)) return -1;
+    for (int i = 0; i < n; ++i) /* field = column */
+      for (int j = 0; j < n; ++j) { /* test = row */
+        size_t c = (size_t)e * n * n + (size_t)i * n + j;
+        kv[c].key = (uint64_t)dofs[n * e + j] * (uint64_t)N + (uint64_t)dofs[n * e + i];
+        kv[c].idx = c;
+        cv[c] = mat[i * n + j];
+      }
+  }
+  qsort(kv, ncoo, sizeof(or_kv), cmp_kv);
+  int32_t *rp = (int32_t *)calloc((size_t)N + 1, sizeof(int32_t));
+  int32_t *ci = (int32_t *)malloc(sizeof(int32_t) * ncoo);
+  double *va = (double *)malloc(sizeof(double) * ncoo);
+  int64_t m = 0;
+  for (size_t c = 0; c < ncoo; ++c) {
+    if (c == 0 || kv[c].key != kv[c - 1].key) {
+      ci[m] = (int32_t)(kv[c].key % (uint64_t)N);
+      va[m] = cv[kv[c].idx];
+      rp[kv[c].key / (uint64_t)N + 1]++;
+      ++m;
+    } else {
+      va[m - 1] += cv[kv[c].idx];
+    }
+  }
+  for (int64_t r = 0; r < N; ++r) rp[r + 1] += rp[r];
+  free(kv);
+  free(cv);
+  free(dofs);
+  free(ee);
+  *nrows = N;
+  *nnz = m;
+  *rowptr = rp;
+  *colidx = (int32_t *)realloc(ci, sizeof(int32_t) * (size_t)(m ? m : 1));
+  *vals = (double *)realloc(va, sizeof(double) * (size_t)(m ? m : 1));
+  return 0;
+}
+
+/* Homogeneous Dirichlet by elimination: A_int = A_full[free, free], free dofs */
+/* renumbered in increasing order.  The reference imposes Dirichlet data with */
+/* Lagrange multipliers instead (operatorLA3d.hpp:21-90); the interior block  */
+/* of its operator matrix is exactly this restriction.                        */
+int or_restrict(int64_t nrows, const int32_t *rowptr, const int32_t *colidx,
+                const double *vals, const uint8_t *fixed, int64_t *nint, int64_t *nnz_int,
+                int32_t **rowptr_int, int32_t **colidx_int, double **vals_int) {
+  int32_t *map = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nrows ? nrows : 1));
+  int64_t ni = 0;
+  for (int64_t r = 0; r < nrows; ++r) map[r] = fixed[r] ? -1 : (int32_t)ni++;
+  int32_t *rp = (int32_t *)calloc((size_t)ni + 1, sizeof(int32_t));
+  int64_t m = 0;
+  for (int64_t r = 0; r < nrows; ++r) {
+    if (map[r] < 0) continue;
+    for (int32_t k = rowptr[r]; k < rowptr[r + 1]; ++k)
+      if (map[colidx[k]] >= 0) ++m;
+  }
+  int32_t *ci = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m ? m : 1));
+  double *va = (double *)malloc(sizeof(double) * (size_t)(m ? m : 1));
+  m = 0;
+  for (int64_t r = 0; r < nrows; ++r) {
+    if (map[r] < 0) continue;
+    for (int32_t k = rowptr[r]; k < rowptr[r + 1]; ++k)
+      if (map[colidx[k]] >= 0) {
+        ci[m] = map[colidx[k]];
+        va[m] = vals[k];
+        ++m;
+      }
+    rp[map[r] + 1] = (int32_t)m;
+  }
+  free(map);
+  *nint = ni;
+  *nnz_int = m;
+  *rowptr_int = rp;
+  *colidx_int = ci;
+  *vals_int = va;
+  return 0;
+}
+
+void or_free(void *p) { free(p); }
+
+/* Epetra_CrsMatrix::Apply semantics: y = A x, row by row */
+void or_spmv(int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *vals,
+             const double *x, double *y) {
+  for (int64_t r = 0; r < n; ++r) {
+    double s = 0.0;
+    for (int32_t k = rowptr[r]; k < rowptr[r + 1]; ++k) s += vals[k] * x[colidx[k]];
+    y[r] = s;
+  }
+}
+
+static double dot(int64_t n, const double *a, const double *b) {
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+/* Jacobi-preconditioned CG (block size 1), the algorithm behind              */
+/* factoryLinearSolver -> belosBlockCG (belosBlockCG.hpp:46-83) with an       */
+/* Ifpack point-Jacobi preconditioner (preconditionerIfpack.cpp:26-63).       */
+/* Stopping test: explicit ||b - A x||_2 / ||b||_2 <= tol (belosBlockCG::solve */
+/* reports success regardless of convergence, so the test is made here).      */
+/* Returns 0 on convergence, 1 if maxit was reached, -1 on breakdown.         */
+int or_pcg(int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *vals,
+           const double *b, double *x, double tol, int maxit, int *iters, double *relres) {
+  double *dinv = (double *)malloc(sizeof(double) * (size_t)n);
+  double *r = (double *)malloc(sizeof(double) * (size_t)n);
+  double *z = (double *)malloc(sizeof(double) * (size_t)n);
+  double *p = (double *)malloc(sizeof(double) * (size_t)n);
+  double *q = (double *)malloc(sizeof(double) * (size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    dinv[i] = 0.0;
+    for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+      if (colidx[k] == i) dinv[i] = 1.0 / vals[k];
+  }
+  double bn = sqrt(dot(n, b, b));
+  if (bn == 0.0) bn = 1.0;
+  int it = 0, status = 1;
+  for (int restart = 0; restart < 8; ++restart) {
+    or_spmv(n, rowptr, colidx, vals, x, q);
+    for (int64_t i = 0; i < n; ++i) r[i] = b[i] - q[i];
+    double rr = dot(n, r, r);
+    if (sqrt(rr) / bn <= tol) {
+      status = 0;
+      break;
+    }
+    for (int64_t i = 0; i < n; ++i) p[i] = z[i] = dinv[i] * r[i];
+    double rz = dot(n, r, z);
+    while (it < maxit) {
+      or_spmv(n, rowptr, colidx, vals, p, q);
+      double pq = dot(n, p, q);
+      if (!(pq > 0.0)) {
+        status = -1;
+        goto done;
+      }
+      double alpha = rz / pq;
+      for (int64_t i = 0; i < n; ++i) {
+        x[i] += alpha * p[i];
+        r[i] -= alpha * q[i];
+        z[i] = dinv[i] * r[i];
+      }
+      ++it;
+      rr = dot(n, r, r);
+      if (sqrt(rr) / bn <= tol) break;
+      double rz2 = dot(n, r, z);
+      double beta = rz2 / rz;
+      rz = rz2;
+      for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+    if (it >= maxit) break;
+  }
+done:
+  or_spmv(n, rowptr, colidx, vals, x, q);
+  for (int64_t i = 0; i < n; ++i) r[i] = b[i] - q[i];
+  *relres = sqrt(dot(n, r, r)) / bn;
+  if (status != -1) status = (*relres <= tol) ? 0 : 1;
+  *iters = it;
+  free(dinv);
+  free(r);
+  free(z);
+  free(p);
+  free(q);
+  return status;
+}
+
+/* Dirichlet mask: dofs lying on boundary triangles whose geoId is selected  */
+/* (the boundary-card selection of bcCardFE/boundCard; a P2 edge dof is fixed */
+/* when its edge is an edge of a selected triangle).                          */
+void or_dirichlet_mask(int order, int64_t nV, int64_t nB, const int32_t *btri,
+                       const int32_t *bgeo, int ngeo, const int32_t *geo_list, int64_t nE,
+                       const int32_t *edge_nodes, uint8_t *fixed) {
+  int64_t N = order == 1 ? nV : nV + nE;
+  memset(fixed, 0, (size_t)N);
+  static const int TE[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+  for (int64_t t = 0; t < nB; ++t) {
+    int sel = 0;
+    for (int g = 0; g < ngeo; ++g) sel |= (bgeo[t] == geo_list[g]);
+    if (!sel) continue;
+    for (int v = 0; v < 3; ++v) fixed[btri[3 * t + v] - 1] = 1;
+    if (order == 2)
+      for (int k = 0; k < 3; ++k) {
+        int32_t a = btri[3 * t + TE[k][0]] - 1, b = btri[3 * t + TE[k][1]] - 1;
+        if (a > b) {
+          int32_t s = a;
+          a = b;
+          b = s;
+        }
+        int64_t lo = 0, hi = nE - 1;
+        while (lo < hi) {
+          int64_t mid = (lo + hi) / 2;
+          if (edge_nodes[2 * mid] < a || (edge_nodes[2 * mid] == a && edge_nodes[2 * mid + 1] < b))
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        fixed[nV + lo] = 1;
+      }
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Barycentric dual-mesh data, restating dual3d<linearTetra>::                */
+/* buildFiniteVolumeDataOnly + buildFiniteVolumeData (dual3d.hpp:606-1025):   */
+/* per tet, 12 interface triangles (edge midpoint, face barycentre, element  */
+/* barycentre) for each face j and edge faceToEdge(j,k); their normals        */
+/* N = (P2-P1) x (P3-P1) are added to edgeN aligned with the edge direction; */
+/* edgeSurf = |edgeN|/2; cellVolume(v) sums |det|/6 of the 24 sub-cells       */
+/* (barycentre, edge midpoint, face barycentre, vertex).                      */
+/* Extension (not in the reference): sigma_el[6e+j] = (N_1+N_2).n_hat / 2,    */
+/* the share of edgeSurf that element e holds for its local edge j, so that   */
+/* sum_e sigma_el = edgeSurf exactly.  Edge direction: lower node id -> higher.*/
+/* ------------------------------------------------------------------------- */
+static const int FACE_V[4][3] = {{0, 2, 1}, {0, 1, 3}, {1, 2, 3}, {0, 3, 2}}; /* faceToPoint */
+static const int FACE_E[4][3] = {{2, 1, 0}, {0, 4, 3}, {1, 5, 4}, {3, 5, 2}}; /* faceToEdge  */
+
+static void cross(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* per element: the aligned interface normal sum for each local edge (6x3) */
+static void elem_dual_normals(const double *P, const int32_t *tv, double *Ne, double *sub_vol) {
+  double BA[3], F[4][3], Em[6][3];
+  for (int d = 0; d < 3; ++d) BA[d] = (P[d] + P[3 + d] + P[6 + d] + P[9 + d]) / 4.0;
+  for (int f = 0; f < 4; ++f)
+    for (int d = 0; d < 3; ++d)
+      F[f][d] = (P[3 * FACE_V[f][0] + d] + P[3 * FACE_V[f][1] + d] + P[3 * FACE_V[f][2] + d]) / 3.0;
+  for (int j = 0; j < 6; ++j)
+    for (int d = 0; d < 3; ++d) Em[j][d] = (P[3 * EDGE_V[j][0] + d] + P[3 * EDGE_V[j][1] + d]) / 2.0;
+  memset(Ne, 0, sizeof(double) * 18);
+  for (int f = 0; f < 4; ++f)
+    for (int k = 0; k < 3; ++k) {
+      int j = FACE_E[f][k];
+      double a[3], b[3], N[3], t[3];
+      for (int d = 0; d < 3; ++d) {
+        a[d] = F[f][d] - Em[j][d];
+        b[d] = BA[d] - Em[j][d];
+      }
+      cross(a, b, N);
+      int lo = EDGE_V[j][0], hi = EDGE_V[j][1];
+      if (tv[lo] > tv[hi]) {
+        int s = lo;
+        lo = hi;
+        hi = s;
+      }
+      for (int d = 0; d < 3; ++d) t[d] = P[3 * hi + d] - P[3 * lo + d];
+      double s = (N[0] * t[0] + N[1] * t[1] + N[2] * t[2]) >= 0.0 ? 1.0 : -1.0;
+      for (int d = 0; d < 3; ++d) Ne[3 * j + d] += s * N[d];
+    }
+  if (sub_vol) {
+    for (int v = 0; v < 4; ++v) sub_vol[v] = 0.0;
+    for (int f = 0; f < 4; ++f)
+      for (int k = 0; k < 3; ++k) {
+        int j = FACE_E[f][k];
+        for (int end = 0; end < 2; ++end) {
+          int v = EDGE_V[j][end];
+          double a[3], b[3], c[3], M[3];
+          for (int d = 0; d < 3; ++d) {
+            a[d] = Em[j][d] - BA[d];
+            b[d] = F[f][d] - BA[d];
+            c[d] = P[3 * v + d] - BA[d];
+          }
+          cross(a, b, M);
+          sub_vol[v] += fabs(M[0] * c[0] + M[1] * c[1] + M[2] * c[2]) / 6.0;
+        }
+      }
+  }
+}
+
+void or_dual_fv(int64_t nV, const double *xyz, int64_t nT, const int32_t *tets, int64_t nE,
+                const int32_t *elem_edge, const int32_t *edge_nodes, double *edge_surf,
+                double *edge_n, double *cell_volume, double *sigma_el) {
+  (void)edge_nodes;
+  double *acc = (double *)calloc(3 * (size_t)nE, sizeof(double));
+  memset(cell_volume, 0, sizeof(double) * (size_t)nV);
+  double P[12], Ne[18], sv[4];
+  int32_t tv[4];
+  for (int64_t e = 0; e < nT; ++e) {
+    for (int v = 0; v < 4; ++v) {
+      tv[v] = tets[4 * e + v] - 1;
+      for (int d = 0; d < 3; ++d) P[3 * v + d] = xyz[3 * tv[v] + d];
+    }
+    elem_dual_normals(P, tv, Ne, sv);
+    for (int j = 0; j < 6; ++j)
+      for (int d = 0; d < 3; ++d) acc[3 * elem_edge[6 * e + j] + d] += Ne[3 * j + d];
+    for (int v = 0; v < 4; ++v) cell_volume[tv[v]] += sv[v];
+  }
+  for (int64_t i = 0; i < nE; ++i) {
+    double nn = sqrt(acc[3 * i] * acc[3 * i] + acc[3 * i + 1] * acc[3 * i + 1] +
+                     acc[3 * i + 2] * acc[3 * i + 2]);
+    edge_surf[i] = 0.5 * nn;
+    for (int d = 0; d < 3; ++d) edge_n[3 * i + d] = acc[3 * i + d] / nn;
+  }
+  if (sigma_el)
+    for (int64_t e = 0; e < nT; ++e) {
+      for (int v = 0; v < 4; ++v) {
+        tv[v] = tets[4 * e + v] - 1;
+        for (int d = 0; d < 3; ++d) P[3 * v + d] = xyz[3 * tv[v] + d];
+      }
+      elem_dual_normals(P, tv, Ne, NULL);
+      for (int j = 0; j < 6; ++j) {
+        const double *nh = &edge_n[3 * elem_edge[6 * e + j]];
+        sigma_el[6 * e + j] = 0.5 * (Ne[3 * j] * nh[0] + Ne[3 * j + 1] * nh[1] + Ne[3 * j + 2] * nh[2]);
+      }
+    }
+  free(acc);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Scharfetter-Gummel (not in the reference; parity by closed forms and FD). */
+/* B(x) = x / (e^x - 1), B(0) = 1; Taylor series for |x| < 1e-2.             */
+/* ------------------------------------------------------------------------- */
+double or_bernoulli(double x) {
+  double ax = fabs(x);
+  if (ax < 1e-2) {
+    double x2 = x * x;
+    /* 1 - x/2 + x^2/12 - x^4/720 + x^6/30240 */
+    return 1.0 - 0.5 * x + x2 / 12.0 * (1.0 - x2 / 60.0 * (1.0 - x2 / 42.0));
+  }
+  if (x > 700.0) return x * exp(-x);
+  return x / expm1(x);
+}
+
+static double bernoulli_d(double x) {
+  /* B'(x) = (e^x - 1 - x e^x) / (e^x - 1)^2 */
+  if (fabs(x) < 1e-2) {
+    double x2 = x * x;
+    return -0.5 + x / 6.0 - x * x2 / 180.0 + x * x2 * x2 / 5040.0;
+  }
+  double em = expm1(x), ex = em + 1.0;
+  return (em - x * ex) / (em * em);
+}
+
+/* Element geometry for P1: vol and grad(lambda) (4x3) */
+static double p1_geom(const double *P, double *G) {
+  double J[9], T[9];
+  or_tet_jacobian(P, J);
+  or_inv3(J, T);
+  double det = or_det3(J);
+  /* grad lambda_1..3 = rows of J^{-1} (columns of J^{-T}); lambda_0 = -sum */
+  for (int a = 0; a < 3; ++a)
+    for (int d = 0; d < 3; ++d) G[3 * (a + 1) + d] = T[3 * a + d];
+  for (int d = 0; d < 3; ++d) G[d] = -(G[3 + d] + G[6 + d] + G[9 + d]);
+  return det / 6.0;
+}
+
+/* Drift-diffusion residual at a given state (phi, n):                        */
+/*  R_phi(v) = sum_T eps_T vol_T grad(lambda_v).grad(phi_h) - rho_T vol_T / 4   */
+/*  R_n(v)   = sum over element-edges (v,w) of s(v) * mu_T sigma_T(edge) / L *  */
+/*             [n_w B(phi_w - phi_v) - n_v B(phi_v - phi_w)]  (flux v -> w)     */
+/* evaluated at non-fixed nodes; loss = sum R_phi^2 + R_n^2.                  */
+double or_dd_loss(int64_t nV, const double *xyz, int64_t nT, const int32_t *tets, int64_t nE,
+                  const int32_t *elem_edge, const int32_t *edge_nodes,
+                  const double *sigma_el, const double *eps, const double *rho,
+                  const double *mu, const double *phi, const double *dens,
+                  const uint8_t *fixed, double *r_phi, double *r_n) {
+  (void)nE;
+  (void)elem_edge;
+  (void)edge_nodes;
+  memset(r_phi, 0, sizeof(double) * (size_t)nV);
+  memset(r_n, 0, sizeof(double) * (size_t)nV);
+  double P[12], G[12];
+  int32_t tv[4];
+  for (int64_t e = 0; e < nT; ++e) {
+    for (int v = 0; v < 4; ++v) {
+      tv[v] = tets[4 * e + v] - 1;
+      for (int d = 0; d < 3; ++d) P[3 * v + d] = xyz[3 * tv[v] + d];
+    }
+    double vol = p1_geom(P, G);
+    double gp[3] = {0, 0, 0};
+    for (int v = 0; v < 4; ++v)
+      for (int d = 0; d < 3; ++d) gp[d] += phi[tv[v]] * G[3 * v + d];
+    for (int v = 0; v < 4; ++v)
+      r_phi[tv[v]] += eps[e] * vol * (G[3 * v] * gp[0] + G[3 * v + 1] * gp[1] + G[3 * v + 2] * gp[2]) -
+                      rho[e] * vol / 4.0;
+    for (int j = 0; j < 6; ++j) {
+      int a = tv[EDGE_V[j][0]], b = tv[EDGE_V[j][1]];
+      double L = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        double t = xyz[3 * a + d] - xyz[3 * b + d];
+        L += t * t;
+      }
+      L = sqrt(L);
+      double c = mu[e] * sigma_el[6 * e + j] / L;
+      double dab = phi[b] - phi[a];
+      double F = c * (dens[b] * or_bernoulli(dab) - dens[a] * or_bernoulli(-dab)); /* a -> b */
+      r_n[a] += F;
+      r_n[b] -= F;
+    }
+  }
+  double loss = 0.0;
+  for (int64_t v = 0; v < nV; ++v) {
+    if (fixed[v]) {
+      r_phi[v] = 0.0;
+      r_n[v] = 0.0;
+      continue;
+    }
+    loss += r_phi[v] * r_phi[v] + r_n[v] * r_n[v];
+  }
+  return loss;
+}
+
+/* d loss / d eps_T and d loss / d mu_T, by the chain rule through the        */
+/* element contributions (the residual is linear in eps and in mu).           */
+void or_dd_grad(int64_t nV, const double *xyz, int64_t nT, const int32_t *tets, int64_t nE,
+                const int32_t *elem_edge, const int32_t *edge_nodes, const double *sigma_el,
+                const double *eps, const double *rho, const double *mu, const double *phi,
+                const double *dens, const uint8_t *fixed, double *g_eps, double *g_mu) {
+  double *rp = (double *)malloc(sizeof(double) * (size_t)nV);
+  double *rn = (double *)malloc(sizeof(double) * (size_t)nV);
+  or_dd_loss(nV, xyz, nT, tets, nE, elem_edge, edge_nodes, sigma_el, eps, rho, mu, phi, dens,
+             fixed, rp, rn);
+  double P[12], G[12];
+  int32_t tv[4];
+  for (int64_t e = 0; e < nT; ++e) {
+    for (int v = 0; v < 4; ++v) {
+      tv[v] = tets[4 * e + v] - 1;
+      for (int d = 0; d < 3; ++d) P[3 * v + d] = xyz[3 * tv[v] + d];
+    }
+    double vol = p1_geom(P, G);
+    double gp[3] = {0, 0, 0};
+    for (int v = 0; v < 4; ++v)
+      for (int d = 0; d < 3; ++d) gp[d] += phi[tv[v]] * G[3 * v + d];
+    double ge = 0.0;
+    for (int v = 0; v < 4; ++v)
+      ge += 2.0 * rp[tv[v]] * vol * (G[3 * v] * gp[0] + G[3 * v + 1] * gp[1] + G[3 * v + 2] * gp[2]);
+    g_eps[e] = ge;
+    double gm = 0.0;
+    for (int j = 0; j < 6; ++j) {
+      int a = tv[EDGE_V[j][0]], b = tv[EDGE_V[j][1]];
+      double L = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        double t = xyz[3 * a + d] - xyz[3 * b + d];
+        L += t * t;
+      }
+      L = sqrt(L);
+      double dab = phi[b] - phi[a];
+      double f = sigma_el[6 * e + j] / L * (dens[b] * or_bernoulli(dab) - dens[a] * or_bernoulli(-dab));
+      gm += 2.0 * (rn[a] - rn[b]) * f;
+    }
+    g_mu[e] = gm;
+  }
+  (void)bernoulli_d;
+  free(rp);
+  free(rn);
+}
+
+/* Adjoint element kernel for dJ/dk: dJ/dk_T = - lam_T^T Khat_T u_T, Khat the */
+/* element stiffness at unit coefficient (opEL3dA with c = 1), u and lam given */
+/* on the full dof numbering (zero on fixed dofs).                            */
+void or_adjoint_grad_k(int order, int64_t nV, const double *xyz, int64_t nT,
+                       const int32_t *tets, const int32_t *dofs, int rule,
+                       const double *u, const double *lam, double *dJdk) {
+  (void)nV;
+  int n = order == 1 ? 4 : 10;
+  double P[12], mat[100];
+  for (int64_t e = 0; e < nT; ++e) {
+    for (int v = 0; v < 4; ++v)
+      for (int d = 0; d < 3; ++d) P[3 * v + d] = xyz[3 * (tets[4 * e + v] - 1) + d];
+    or_element_matrix(OR_OP_STIFF, order, P, 1.0, 0.0, rule, 1, mat);
+    double s = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) s += lam[dofs[n * e + j]] * mat[i * n + j] * u[dofs[n * e + i]];
+    dJdk[e] = -s;
+  }
+}
