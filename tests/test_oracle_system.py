@@ -239,3 +239,19 @@ def test_dd_grad_central_differences():
             lm, _, _ = O.dd_loss(m, d, a["eps"], rho, a["mu"], phi, dens, fixed, ee, en)
             fd = (lp - lm) / (2 * h)
             assert abs(fd - g[e]) <= 1e-6 * max(1.0, abs(g[e])), (which, e, fd, g[e])
+
+
+def test_reference_mesh_fixtures_load_and_assemble():
+    import glob
+    import os
+    for path in sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "meshes",
+                                              "*.msh"))):
+        m = W.read_gmsh22(path)
+        P = m.xyz[m.tets - 1]
+        det = np.linalg.det(np.stack([P[:, 1] - P[:, 0], P[:, 2] - P[:, 0], P[:, 3] - P[:, 0]],
+                                     axis=2))
+        assert det.min() > 0, path
+        rp, ci, va = O.assemble(2, m, O.OP_STIFF, rule_stiff=4)
+        N = rp.size - 1
+        A = sp.csr_matrix((va, ci, rp), shape=(N, N))
+        assert np.abs(A @ np.ones(N)).max() < 1e-13
