@@ -6,15 +6,16 @@
 //     as geoMapInterface::getGradientDet), J^{-1} by cofactors (tensor3d::
 //     computeInverse), and the 7 per-element factors the local matrix needs:
 //     c det (J^{-1} J^{-T})_k for the 6 symmetric pairs k, and c_mass det.
-//  2. k_asm_rows (one lane per CSR row, 64 rows per workgroup): each lane walks
-//     the (element, local basis l) incidences of its row, forms row l of the
-//     element matrix as sum_k factor_k tab[k][l][m] (tab = the quadrature of
-//     integratorOEL3d_STD contracted once on the reference element, held in
-//     constant memory and read with scalar loads because l is uniform across
-//     the wave), and adds it into the row's entries staged in LDS.  A row is
-//     owned by one lane and its incidences are visited in a fixed order, so the
-//     sums are deterministic without atomics.  The workgroup then writes its
-//     contiguous CSR value range with coalesced stores.
+//  2. k_asm_batched (one wavefront per schedule block of consecutive CSR rows,
+//     the block's entries staged in LDS): the block's (element, local basis l)
+//     incidences come in precomputed batches of <= 64 that share l and hit 64
+//     distinct rows.  Each lane forms row l of its element's matrix as
+//     sum_k factor_k tab[k][l][m] (tab = the quadrature of integratorOEL3d_STD
+//     contracted once on the reference element, in constant memory, read with
+//     scalar loads because l is uniform in the batch) and adds it into its
+//     row's LDS entries.  Lanes of a batch never collide and batches run in a
+//     fixed order, so the sums are deterministic without atomics.  The block's
+//     contiguous CSR value range is then written with coalesced stores.
 #include <cstring>
 
 #include "mg_internal.h"
@@ -126,69 +127,113 @@ __global__ void k_geom(int64_t nT, const int32_t *__restrict__ tets, const doubl
   for (int k = 0; k < 8; ++k) dst[k] = out[k];
 }
 
+template <typename T>
+struct Geom8 {
+  T v[8];
+};
+
+template <typename T>
+__device__ __forceinline__ Geom8<T> load_geom(const T *__restrict__ geom, int32_t el) {
+  Geom8<T> g;
+  const T *src = geom + 8 * (size_t)(el >> 4);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) g.v[k] = src[k];
+  return g;
+}
+
+// One wavefront per schedule block.  Two-deep software pipeline: while batch b
+// is contracted and added, the geometry of batch b+1 and the records of batch
+// b+2 are in flight.
 template <int NL, typename T, bool MASS, typename Rec>
-__global__ __launch_bounds__(64) void k_asm_rows(int32_t row_begin, int32_t row_end,
-                                                 const int32_t *__restrict__ rowptr,
-                                                 const int32_t *__restrict__ inc_ptr,
-                                                 const Rec *__restrict__ rec,
-                                                 const T *__restrict__ geom, T *__restrict__ vals) {
+__global__ __launch_bounds__(64) void k_asm_batched(const AsmBlock *__restrict__ blocks,
+                                                    const AsmBatch *__restrict__ batches,
+                                                    const Rec *__restrict__ rec,
+                                                    const int32_t *__restrict__ rowptr,
+                                                    const T *__restrict__ geom,
+                                                    T *__restrict__ vals, int slots_cap) {
   extern __shared__ __align__(16) char lds_raw[];
   T *acc = reinterpret_cast<T *>(lds_raw);
-  const int r0 = row_begin + blockIdx.x * 64;
-  const int r1 = min(r0 + 64, row_end);
-  const int s0 = rowptr[r0];
-  const int ns = rowptr[r1] - s0;
-  for (int s = threadIdx.x; s < ns; s += 64) acc[s] = (T)0;
+  int32_t *lrp = reinterpret_cast<int32_t *>(acc + slots_cap + 1);
+  const int lane = threadIdx.x;
+  const AsmBlock B = blocks[blockIdx.x];
+  const int s0 = rowptr[B.row_begin];
+  const int ns = rowptr[B.row_end] - s0;
+  for (int s = lane; s < ns; s += 64) acc[s] = (T)0;
+  for (int i = lane; i <= B.row_end - B.row_begin; i += 64) lrp[i] = rowptr[B.row_begin + i] - s0;
   __syncthreads();
-  const int r = r0 + threadIdx.x;
-  if (r < r1) {
-    const int base = rowptr[r] - s0;
-    int i = inc_ptr[r];
-    const int iend = inc_ptr[r + 1];
-    for (int l = 0; l < NL; ++l) {
-      while (i < iend) {
-        const Rec rc = rec[i];
-        if ((rc.el & 15) != l) break;
-        const T *g = geom + 8 * (size_t)(rc.el >> 4);
-        T gk[7];
-#pragma unroll
-        for (int k = 0; k < 7; ++k) gk[k] = g[k];
-        T v[NL];
-#pragma unroll
-        for (int m = 0; m < NL; ++m) {
-          T s = gk[0] * tab<T>((0 * NL + l) * NL + m);
-#pragma unroll
-          for (int k = 1; k < 6; ++k) s += gk[k] * tab<T>((k * NL + l) * NL + m);
-          if (MASS) s += gk[6] * tab<T>((6 * NL + l) * NL + m);
-          v[m] = s;
-        }
-        T a[NL];
-#pragma unroll
-        for (int m = 0; m < NL; ++m) a[m] = rc.pos[m] != 0xFF ? acc[base + rc.pos[m]] : (T)0;
-#pragma unroll
-        for (int m = 0; m < NL; ++m)
-          if (rc.pos[m] != 0xFF) acc[base + rc.pos[m]] = a[m] + v[m];
-        ++i;
-      }
+  const int dummy = slots_cap;  // target of eliminated columns, never written out
+
+  int b = B.batch_begin;
+  const int bend = B.batch_end;
+  AsmBatch m0{0, 0};
+  if (b < bend) m0 = batches[b];
+  Rec r0{}, r1{};
+  int c0 = m0.meta >> 8, c1 = 0;
+  if (lane < c0) r0 = rec[m0.offset + lane];
+  Geom8<T> g0{};
+  if (lane < c0) g0 = load_geom(geom, r0.el);
+  AsmBatch m1{0, 0};
+  if (b + 1 < bend) {
+    m1 = batches[b + 1];
+    c1 = m1.meta >> 8;
+    if (lane < c1) r1 = rec[m1.offset + lane];
+  }
+  for (; b < bend; ++b) {
+    // prefetch: geometry of b+1, records of b+2
+    Geom8<T> g1{};
+    if (lane < c1) g1 = load_geom(geom, r1.el);
+    Rec r2{};
+    int c2 = 0;
+    if (b + 2 < bend) {
+      const AsmBatch m2 = batches[b + 2];
+      c2 = m2.meta >> 8;
+      if (lane < c2) r2 = rec[m2.offset + lane];
     }
+    const int l = __builtin_amdgcn_readfirstlane(m0.meta & 0xFF);
+    if (lane < c0) {
+      const int base = lrp[r0.row];
+      T v[NL];
+#pragma unroll
+      for (int m = 0; m < NL; ++m) {
+        T s = g0.v[0] * tab<T>((0 * NL + l) * NL + m);
+#pragma unroll
+        for (int k = 1; k < 6; ++k) s += g0.v[k] * tab<T>((k * NL + l) * NL + m);
+        if (MASS) s += g0.v[6] * tab<T>((6 * NL + l) * NL + m);
+        v[m] = s;
+      }
+      int idx[NL];
+#pragma unroll
+      for (int m = 0; m < NL; ++m) idx[m] = r0.pos[m] == 0xFF ? dummy : base + r0.pos[m];
+      T a[NL];
+#pragma unroll
+      for (int m = 0; m < NL; ++m) a[m] = acc[idx[m]];
+#pragma unroll
+      for (int m = 0; m < NL; ++m) acc[idx[m]] = a[m] + v[m];
+    }
+    // rotate the pipeline
+    m0 = m1;
+    c0 = c1;
+    r0 = r1;
+    g0 = g1;
+    c1 = c2;
+    r1 = r2;
+    if (b + 2 < bend) m1 = batches[b + 2];
   }
   __syncthreads();
-  for (int s = threadIdx.x; s < ns; s += 64) vals[s0 + s] = acc[s];
+  for (int s = lane; s < ns; s += 64) vals[s0 + s] = acc[s];
 }
 
 template <int NL, typename T, typename Rec>
 static void launch_rows(Ctx *c, System &S, bool mass, const T *geom, T *vals) {
-  for (const RowSegment &sg : S.segs) {
-    const int nblk = (sg.row_end - sg.row_begin + 63) / 64;
-    const size_t lds = sizeof(T) * (size_t)sg.lds_slots;
-    if (mass)
-      k_asm_rows<NL, T, true, Rec><<<nblk, 64, lds, c->stream>>>(
-          sg.row_begin, sg.row_end, S.rowptr, S.inc_ptr, (const Rec *)S.inc_rec, geom, vals);
-    else
-      k_asm_rows<NL, T, false, Rec><<<nblk, 64, lds, c->stream>>>(
-          sg.row_begin, sg.row_end, S.rowptr, S.inc_ptr, (const Rec *)S.inc_rec, geom, vals);
-    MG_HIP(hipGetLastError());
-  }
+  const size_t lds = sizeof(T) * (size_t)(S.asm_slots + 1) + sizeof(int32_t) * (S.asm_rows + 1);
+  const int nblk = (int)S.n_blocks;
+  if (mass)
+    k_asm_batched<NL, T, true, Rec><<<nblk, 64, lds, c->stream>>>(
+        S.blocks, S.batches, (const Rec *)S.brec, S.rowptr, geom, vals, S.asm_slots);
+  else
+    k_asm_batched<NL, T, false, Rec><<<nblk, 64, lds, c->stream>>>(
+        S.blocks, S.batches, (const Rec *)S.brec, S.rowptr, geom, vals, S.asm_slots);
+  MG_HIP(hipGetLastError());
 }
 
 static bool rule_ok(int r) { return r == 1 || r == 4 || r == 11 || r == 15; }
@@ -213,9 +258,9 @@ static void assemble_t(Ctx *c, int sys, const mg_operator *op, int prec) {
   MG_HIP(hipGetLastError());
   T *vals = prec == MG_PREC_F64 ? (T *)S.vals64 : (T *)S.vals32;
   if (c->nloc == 10)
-    launch_rows<10, T, RecP2>(c, S, mass, geom, vals);
+    launch_rows<10, T, BRecP2>(c, S, mass, geom, vals);
   else
-    launch_rows<4, T, RecP1>(c, S, mass, geom, vals);
+    launch_rows<4, T, BRecP1>(c, S, mass, geom, vals);
 }
 
 // ---- adjoint element kernel ---------------------------------------------------

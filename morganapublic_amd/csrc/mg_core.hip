@@ -18,6 +18,15 @@ void dfree(Ctx *c, void *p) {
     }
 }
 
+void free_system(Ctx *c, System &S) {
+  for (void *p : {(void *)S.rowptr, (void *)S.colidx, (void *)S.diagpos, (void *)S.slot,
+                  (void *)S.inc_ptr, S.inc_rec, (void *)S.vals64, (void *)S.vals32,
+                  (void *)S.dinv64, (void *)S.dinv32, (void *)S.blocks, (void *)S.batches,
+                  S.brec})
+    dfree(c, p);
+  S = System();
+}
+
 void *scratch(Ctx *c, size_t bytes) {
   if (bytes > c->tmp_bytes) {
     dfree(c, c->tmp);
@@ -195,12 +204,7 @@ __global__ void k_btri_check(int64_t nB, int64_t nV, const int32_t *__restrict__
 
 static void reset_derived(Ctx *c) {
   for (int s = 0; s < 2; ++s) {
-    System &S = c->sys[s];
-    for (void *p : {(void *)S.rowptr, (void *)S.colidx, (void *)S.diagpos, (void *)S.slot,
-                    (void *)S.inc_ptr, S.inc_rec, (void *)S.vals64, (void *)S.vals32,
-                    (void *)S.dinv64, (void *)S.dinv32})
-      dfree(c, p);
-    S = System();
+    free_system(c, c->sys[s]);
   }
   for (void *p : {(void *)c->edge_keys, (void *)c->edge_nodes, (void *)c->elem_edge,
                   (void *)c->dofs, (void *)c->fixed, (void *)c->sysmap, (void *)c->edge_surf,
@@ -350,12 +354,7 @@ int mg_dofmap_build(mg_ctx *c, int order, int64_t *n_dofs, int64_t *n_edges) {
   MG_CHECK(order == 1 || order == 2, MG_ERR_ARG, "dofmap: order must be 1 or 2");
   MG_CHECK(c->nT > 0, MG_ERR_STATE, "dofmap: no mesh");
   for (int s = 0; s < 2; ++s) {
-    System &S = c->sys[s];
-    for (void *p : {(void *)S.rowptr, (void *)S.colidx, (void *)S.diagpos, (void *)S.slot,
-                    (void *)S.inc_ptr, S.inc_rec, (void *)S.vals64, (void *)S.vals32,
-                    (void *)S.dinv64, (void *)S.dinv32})
-      dfree(c, p);
-    S = System();
+    free_system(c, c->sys[s]);
   }
   dfree(c, c->fixed);
   dfree(c, c->sysmap);

@@ -12,16 +12,10 @@
 
 namespace mg {
 
-// One contiguous range of 64-row blocks launched with one LDS size.
-struct RowSegment {
-  int32_t row_begin;
-  int32_t row_end;
-  int32_t lds_slots;  // max CSR slots any 64-row block of the range owns
-};
-
 // Incidence record: element e sees row dof as its local basis l; pos[m] is
 // the position, inside that row, of the column dof of local basis m (0xFF when
-// the column is eliminated).  One 16-byte load for P2, 8 bytes for P1.
+// the column is eliminated).  Built once per pattern, then reordered into the
+// assembly schedule below.
 struct alignas(16) RecP2 {
   int32_t el;  // e << 4 | l
   uint8_t pos[10];
@@ -32,6 +26,30 @@ struct alignas(8) RecP1 {
   uint8_t pos[4];
 };
 
+// Assembly schedule.  Rows are cut into blocks whose CSR entries fit the LDS
+// budget; the incidences of a block are packed into batches of at most 64 that
+// share the local basis l (so the element tensors are read with scalar loads)
+// and come from 64 distinct rows (so their LDS updates never collide).  One
+// wavefront runs one block, batch after batch, in a fixed order.
+struct AsmBlock {
+  int32_t row_begin, row_end, batch_begin, batch_end;
+};
+struct AsmBatch {
+  int32_t offset;  // first record of the batch
+  int32_t meta;    // l | count << 8
+};
+struct alignas(16) BRecP2 {
+  int32_t el;    // e << 4 | l
+  uint16_t row;  // row index inside the block
+  uint8_t pos[10];
+};
+struct alignas(4) BRecP1 {
+  int32_t el;
+  uint16_t row;
+  uint8_t pos[4];
+  uint8_t pad[2];
+};
+
 struct System {
   bool built = false;
   int64_t nrows = 0, nnz = 0, n_inc = 0;
@@ -40,9 +58,14 @@ struct System {
   int32_t *colidx = nullptr;   // nnz
   int32_t *diagpos = nullptr;  // nrows, CSR index of the diagonal entry
   int32_t *slot = nullptr;     // nT * nloc * nloc element -> CSR slot map
-  int32_t *inc_ptr = nullptr;  // nrows + 1
-  void *inc_rec = nullptr;     // n_inc records (RecP1 or RecP2)
-  std::vector<RowSegment> segs;
+  int32_t *inc_ptr = nullptr;  // nrows + 1 (row -> incidences)
+  void *inc_rec = nullptr;     // n_inc records (RecP1 or RecP2), sorted by (row, l, e)
+  // assembly schedule
+  int32_t asm_slots = 0, asm_rows = 0;  // LDS entries / rows per block (upper bounds)
+  int64_t n_blocks = 0, n_batches = 0;
+  AsmBlock *blocks = nullptr;
+  AsmBatch *batches = nullptr;
+  void *brec = nullptr;  // n_inc BRecP1 / BRecP2 in batch order
   double *vals64 = nullptr;
   float *vals32 = nullptr;
   bool has64 = false, has32 = false;
@@ -128,6 +151,7 @@ T *dalloc(Ctx *c, size_t n) {
   return static_cast<T *>(p);
 }
 void dfree(Ctx *c, void *p);
+void free_system(Ctx *c, System &S);  // releases a system's buffers and resets it
 void *scratch(Ctx *c, size_t bytes);  // context-owned reusable temporary
 
 inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }

@@ -2,14 +2,15 @@
 // inside an FP64 iterative refinement).
 //
 // One CG iteration is three kernels, all grid-stride over a fixed grid:
-//   k_spmv_pq : q = A p and the block partials of p.q
-//   k_update  : x += alpha p, r -= alpha q, partials of r.D^{-1}r and r.r
-//   k_pupdate : p = D^{-1} r + beta p
-// Scalars (alpha, beta, r.z, convergence flag) live in device memory: the last
-// workgroup to finish a reduction (ticket counter) sums the per-workgroup
-// partials in a fixed order and computes the next scalar, so the host only
-// reads the state every `check_every` iterations and no reduction uses float
-// atomics.  Once converged, the queued kernels return at entry.
+//   k_spmv_pq : q = A p and the per-workgroup partials of p.q
+//   k_update  : alpha, x += alpha p, r -= alpha q, partials of r.D^{-1}r, r.r
+//   k_pupdate : convergence test, beta, p = D^{-1} r + beta p
+// Every workgroup of a kernel first sums the partials the previous kernel
+// wrote, in the same fixed order, so all workgroups hold bit-identical
+// scalars without communicating inside a kernel (a device-scope fence per
+// workgroup would write back the L2 of its XCD each time).  The host reads the
+// state every `check_every` iterations; once converged, queued kernels return
+// at entry.  No reduction uses float atomics.
 #include <cmath>
 
 #include "mg_internal.h"
@@ -19,20 +20,21 @@ namespace mg {
 static constexpr int kThreads = 256;
 static constexpr int kMaxBlocks = 2048;
 
+// Host-visible solver state; written by block 0 of a kernel or by a one-block
+// finalize kernel, read by later kernels (kernel boundaries order the writes).
 struct PcgState {
-  double rz, alpha, beta, rr, tol2, pq, bb;
-  int iters, maxit, done, status;
-  unsigned ticket[4];
+  double tol2, rr, rz, pq, bb;
+  int maxit, iters, done, status;
 };
 
-__device__ __forceinline__ double ld_coherent(const double *p) {
-  unsigned long long v = __hip_atomic_load(reinterpret_cast<const unsigned long long *>(p),
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double((long long)v);
-}
+// Per-workgroup partial sums of the reductions (kMaxBlocks each).
+struct Partials {
+  double *pq, *rz0, *rz1, *rr, *tmp;
+};
 
-// deterministic block sum of one double per thread (fixed tree)
-__device__ __forceinline__ double block_sum(double v, double *sh) {
+// Every thread of the block gets the same total of one value per thread
+// (fixed shuffle tree + fixed order over the waves).
+__device__ __forceinline__ double block_sum_all(double v, double *sh) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
@@ -40,30 +42,18 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   if (ln == 0) sh[w] = v;
   __syncthreads();
   double t = 0.0;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
-  return t;  // valid in thread 0
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
+  __syncthreads();
+  return t;
 }
 
-// Publishes this block's partial(s); returns true in the block that finished last.
-__device__ __forceinline__ bool last_block(unsigned *ticket, int *sh_flag) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned t = atomicAdd(ticket, 1u);
-    *sh_flag = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  const bool last = *sh_flag;
-  if (last) __threadfence();
-  return last;
-}
-
-// sum of partial[0 .. n) in a fixed order by the whole block
-__device__ __forceinline__ double reduce_partials(const double *partial, int n, double *sh) {
+// Total of the n partials a previous kernel wrote.  Every workgroup computes it
+// in the same order, so all of them obtain the same bits without any
+// cross-workgroup communication inside the kernel.
+__device__ __forceinline__ double total(const double *__restrict__ partial, int n, double *sh) {
   double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) v += ld_coherent(partial + i);
-  return block_sum(v, sh);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v += partial[i];
+  return block_sum_all(v, sh);
 }
 
 template <typename T, int LPR>
@@ -91,15 +81,15 @@ __global__ __launch_bounds__(kThreads) void k_spmv(int n, const int32_t *__restr
   }
 }
 
+// q = A p; partial p.q
 template <typename T, int LPR>
 __global__ __launch_bounds__(kThreads) void k_spmv_pq(int n, const int32_t *__restrict__ rowptr,
                                                       const int32_t *__restrict__ colidx,
                                                       const T *__restrict__ vals,
                                                       const T *__restrict__ p, T *__restrict__ q,
-                                                      double *__restrict__ partial,
-                                                      PcgState *__restrict__ st) {
+                                                      double *__restrict__ pq_part,
+                                                      const PcgState *__restrict__ st) {
   __shared__ double sh[kThreads / 64];
-  __shared__ int flag;
   if (st->done) return;
   const int lane = threadIdx.x % LPR;
   const int stride = gridDim.x * (kThreads / LPR);
@@ -111,88 +101,97 @@ __global__ __launch_bounds__(kThreads) void k_spmv_pq(int n, const int32_t *__re
       acc += (double)(s * p[row]);
     }
   }
-  const double b = block_sum(acc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = b;
-  if (last_block(&st->ticket[0], &flag)) {
-    const double pq = reduce_partials(partial, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      st->ticket[0] = 0;
-      st->pq = pq;
-      if (!(pq > 0.0) || !isfinite(pq)) {
-        st->status = MG_ERR_BREAKDOWN;
-        st->done = 1;
-      } else {
-        st->alpha = st->rz / pq;
-      }
-    }
-  }
+  const double b = block_sum_all(acc, sh);
+  if (threadIdx.x == 0) pq_part[blockIdx.x] = b;
 }
 
+// alpha = rz / pq; x += alpha p; r -= alpha q; partials r.D^{-1}r and r.r
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_update(int n, T *__restrict__ x, T *__restrict__ r,
                                                      const T *__restrict__ p,
                                                      const T *__restrict__ q,
                                                      const T *__restrict__ dinv,
-                                                     double *__restrict__ partial,
+                                                     const double *__restrict__ pq_part, int npq,
+                                                     const double *__restrict__ rz_prev, int nprev,
+                                                     double *__restrict__ rz_out,
+                                                     double *__restrict__ rr_out,
                                                      PcgState *__restrict__ st) {
   __shared__ double sh[kThreads / 64];
-  __shared__ int flag;
   if (st->done) return;
-  const T alpha = (T)st->alpha;
-  double rz = 0.0, rr = 0.0;
+  const double pq = total(pq_part, npq, sh);
+  const double rz = total(rz_prev, nprev, sh);
+  if (!(pq > 0.0) || !isfinite(pq)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->pq = pq;
+      st->status = MG_ERR_BREAKDOWN;
+      st->done = 1;
+    }
+    return;
+  }
+  const T alpha = (T)(rz / pq);
+  double a1 = 0.0, a2 = 0.0;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
     x[i] += alpha * p[i];
     const T ri = r[i] - alpha * q[i];
     r[i] = ri;
-    rz += (double)(ri * (dinv[i] * ri));
-    rr += (double)(ri * ri);
+    a1 += (double)(ri * (dinv[i] * ri));
+    a2 += (double)(ri * ri);
   }
-  const double b1 = block_sum(rz, sh);
-  const double b2 = block_sum(rr, sh);
+  const double b1 = block_sum_all(a1, sh);
+  const double b2 = block_sum_all(a2, sh);
   if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = b1;
-    partial[2 * blockIdx.x + 1] = b2;
-  }
-  if (last_block(&st->ticket[1], &flag)) {
-    double v1 = 0.0, v2 = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
-      v1 += ld_coherent(partial + 2 * i);
-      v2 += ld_coherent(partial + 2 * i + 1);
-    }
-    const double s1 = block_sum(v1, sh);
-    const double s2 = block_sum(v2, sh);
-    if (threadIdx.x == 0) {
-      st->ticket[1] = 0;
-      st->iters += 1;
-      st->rr = s2;
-      if (!isfinite(s2)) {
-        st->status = MG_ERR_BREAKDOWN;
-        st->done = 1;
-      } else if (s2 <= st->tol2) {
-        st->status = MG_OK;
-        st->done = 1;
-      } else if (st->iters >= st->maxit) {
-        st->status = MG_ERR_NOCONV;
-        st->done = 1;
-      }
-      st->beta = s1 / st->rz;
-      st->rz = s1;
-    }
+    rz_out[blockIdx.x] = b1;
+    rr_out[blockIdx.x] = b2;
   }
 }
 
+// convergence test on r.r; beta = rz_new / rz; p = D^{-1} r + beta p
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_pupdate(int n, const T *__restrict__ r,
                                                       T *__restrict__ p,
                                                       const T *__restrict__ dinv,
-                                                      const PcgState *__restrict__ st) {
+                                                      const double *__restrict__ rz_new_part,
+                                                      const double *__restrict__ rz_part,
+                                                      const double *__restrict__ rr_part, int np,
+                                                      int iter, PcgState *__restrict__ st) {
+  __shared__ double sh[kThreads / 64];
   if (st->done) return;
-  const T beta = (T)st->beta;
+  const double rz_new = total(rz_new_part, np, sh);
+  const double rz = total(rz_part, np, sh);
+  const double rr = total(rr_part, np, sh);
+  const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+  if (first) {
+    st->iters = iter + 1;
+    st->rr = rr;
+    st->rz = rz_new;
+  }
+  if (!isfinite(rr)) {
+    if (first) {
+      st->status = MG_ERR_BREAKDOWN;
+      st->done = 1;
+    }
+    return;
+  }
+  if (rr <= st->tol2) {
+    if (first) {
+      st->status = MG_OK;
+      st->done = 1;
+    }
+    return;
+  }
+  if (iter + 1 >= st->maxit) {
+    if (first) {
+      st->status = MG_ERR_NOCONV;
+      st->done = 1;
+    }
+    return;
+  }
+  const T beta = (T)(rz_new / rz);
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
     p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
-// r = b - A x (or r = b when x is null), p = D^{-1} r; st->rz = r.p, st->rr = r.r
+// r = b - A x (or r = b when x is null), p = D^{-1} r; partials r.p, r.r
 template <typename T, int LPR>
 __global__ __launch_bounds__(kThreads) void k_start(int n, const int32_t *__restrict__ rowptr,
                                                     const int32_t *__restrict__ colidx,
@@ -200,13 +199,12 @@ __global__ __launch_bounds__(kThreads) void k_start(int n, const int32_t *__rest
                                                     const T *__restrict__ x,
                                                     const T *__restrict__ b, T *__restrict__ r,
                                                     T *__restrict__ p, const T *__restrict__ dinv,
-                                                    double *__restrict__ partial,
-                                                    PcgState *__restrict__ st) {
+                                                    double *__restrict__ rz_out,
+                                                    double *__restrict__ rr_out) {
   __shared__ double sh[kThreads / 64];
-  __shared__ int flag;
   const int lane = threadIdx.x % LPR;
   const int stride = gridDim.x * (kThreads / LPR);
-  double rz = 0.0, rr = 0.0;
+  double a1 = 0.0, a2 = 0.0;
   for (int row = blockIdx.x * (kThreads / LPR) + threadIdx.x / LPR; row < n; row += stride) {
     T ax = (T)0;
     if (x) ax = row_dot<T, LPR>(row, lane, rowptr, colidx, vals, x);
@@ -215,35 +213,35 @@ __global__ __launch_bounds__(kThreads) void k_start(int n, const int32_t *__rest
       const T zi = dinv[row] * ri;
       r[row] = ri;
       p[row] = zi;
-      rz += (double)(ri * zi);
-      rr += (double)(ri * ri);
+      a1 += (double)(ri * zi);
+      a2 += (double)(ri * ri);
     }
   }
-  const double b1 = block_sum(rz, sh);
-  const double b2 = block_sum(rr, sh);
+  const double b1 = block_sum_all(a1, sh);
+  const double b2 = block_sum_all(a2, sh);
   if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = b1;
-    partial[2 * blockIdx.x + 1] = b2;
-  }
-  if (last_block(&st->ticket[2], &flag)) {
-    double v1 = 0.0, v2 = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
-      v1 += ld_coherent(partial + 2 * i);
-      v2 += ld_coherent(partial + 2 * i + 1);
-    }
-    const double s1 = block_sum(v1, sh);
-    const double s2 = block_sum(v2, sh);
-    if (threadIdx.x == 0) {
-      st->ticket[2] = 0;
-      st->rz = s1;
-      st->rr = s2;
-      st->done = (s2 <= st->tol2) ? 1 : 0;
-      st->status = MG_OK;
-    }
+    rz_out[blockIdx.x] = b1;
+    rr_out[blockIdx.x] = b2;
   }
 }
 
-// FP64 residual r = b - A x with st->rr = r.r (refinement and final check)
+// one workgroup: st->rr (and st->done) from the partials of k_start
+__global__ void k_start_finalize(const double *__restrict__ rz_part,
+                                 const double *__restrict__ rr_part, int np,
+                                 PcgState *__restrict__ st) {
+  __shared__ double sh[kThreads / 64];
+  const double rz = total(rz_part, np, sh);
+  const double rr = total(rr_part, np, sh);
+  if (threadIdx.x == 0) {
+    st->rz = rz;
+    st->rr = rr;
+    st->iters = 0;
+    st->status = MG_OK;
+    st->done = (rr <= st->tol2) ? 1 : 0;
+  }
+}
+
+// FP64 residual r = b - A x; partial r.r
 template <int LPR>
 __global__ __launch_bounds__(kThreads) void k_resid64(int n, const int32_t *__restrict__ rowptr,
                                                       const int32_t *__restrict__ colidx,
@@ -251,10 +249,8 @@ __global__ __launch_bounds__(kThreads) void k_resid64(int n, const int32_t *__re
                                                       const double *__restrict__ x,
                                                       const double *__restrict__ b,
                                                       double *__restrict__ r,
-                                                      double *__restrict__ partial,
-                                                      PcgState *__restrict__ st) {
+                                                      double *__restrict__ part) {
   __shared__ double sh[kThreads / 64];
-  __shared__ int flag;
   const int lane = threadIdx.x % LPR;
   const int stride = gridDim.x * (kThreads / LPR);
   double rr = 0.0;
@@ -266,33 +262,34 @@ __global__ __launch_bounds__(kThreads) void k_resid64(int n, const int32_t *__re
       rr += ri * ri;
     }
   }
-  const double s = block_sum(rr, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-  if (last_block(&st->ticket[3], &flag)) {
-    const double t = reduce_partials(partial, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      st->ticket[3] = 0;
-      st->rr = t;
-    }
-  }
+  const double s = block_sum_all(rr, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-__global__ void k_dot_self(int n, const double *__restrict__ v, double *__restrict__ partial,
-                           PcgState *__restrict__ st) {
+__global__ void k_dot_self(int n, const double *__restrict__ v, double *__restrict__ part) {
   __shared__ double sh[kThreads / 64];
-  __shared__ int flag;
   double a = 0.0;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
     a += v[i] * v[i];
-  const double s = block_sum(a, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-  if (last_block(&st->ticket[3], &flag)) {
-    const double t = reduce_partials(partial, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      st->ticket[3] = 0;
-      st->bb = t;
-    }
-  }
+  const double s = block_sum_all(a, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// one workgroup: folds the first n_from partials into slot 0 and zeroes the
+// slots up to n_to, so a reader summing n_to partials obtains the same total
+__global__ void k_start_partials_to(double *__restrict__ part, int n_from, int n_to) {
+  __shared__ double sh[kThreads / 64];
+  const double t = total(part, n_from, sh);
+  __syncthreads();
+  for (int i = threadIdx.x; i < (n_from > n_to ? n_from : n_to); i += blockDim.x)
+    part[i] = i == 0 ? t : 0.0;
+}
+
+// one workgroup: *out = total of np partials
+__global__ void k_finalize_sum(const double *__restrict__ part, int np, double *__restrict__ out) {
+  __shared__ double sh[kThreads / 64];
+  const double t = total(part, np, sh);
+  if (threadIdx.x == 0) *out = t;
 }
 
 template <typename T>
@@ -374,13 +371,13 @@ struct Workspace {
   double *x64, *r64;
   float *b32, *x32, *r32, *p32, *q32;
   double *p64, *q64, *rr64;
-  double *partial;
+  Partials part;
   PcgState *st;
 };
 
 static Workspace workspace(Ctx *c, int64_t n) {
-  size_t need = sizeof(double) * 5 * n + sizeof(float) * 5 * n + sizeof(double) * 2 * kMaxBlocks +
-                sizeof(PcgState) + 16 * 256;
+  size_t need = sizeof(double) * 5 * n + sizeof(float) * 5 * n + sizeof(double) * 5 * kMaxBlocks +
+                sizeof(PcgState) + 20 * 256;
   if (c->solve_ws_bytes < need) {
     dfree(c, c->solve_ws);
     c->solve_ws = dalloc<char>(c, need);
@@ -403,7 +400,11 @@ static Workspace workspace(Ctx *c, int64_t n) {
   w.r32 = (float *)take(sizeof(float) * n);
   w.p32 = (float *)take(sizeof(float) * n);
   w.q32 = (float *)take(sizeof(float) * n);
-  w.partial = (double *)take(sizeof(double) * 2 * kMaxBlocks);
+  w.part.pq = (double *)take(sizeof(double) * kMaxBlocks);
+  w.part.rz0 = (double *)take(sizeof(double) * kMaxBlocks);
+  w.part.rz1 = (double *)take(sizeof(double) * kMaxBlocks);
+  w.part.rr = (double *)take(sizeof(double) * kMaxBlocks);
+  w.part.tmp = (double *)take(sizeof(double) * kMaxBlocks);
   w.st = (PcgState *)take(sizeof(PcgState));
   return w;
 }
@@ -439,19 +440,33 @@ static PcgState pcg_core(Ctx *c, System &S, const T *vals, const T *dinv, const 
   init_state(c, w.st, tol2, maxit);
   const int gr = grid_rows(n, LPR), gv = grid_vec(n);
   if (x_is_zero) MG_HIP(hipMemsetAsync(x, 0, sizeof(T) * n, c->stream));
+  Partials &P = w.part;
+  // k_start writes the r.z partials where iteration 0 reads its "previous" r.z
   k_start<T, LPR><<<gr, kThreads, 0, c->stream>>>(n, S.rowptr, S.colidx, vals,
-                                                  x_is_zero ? nullptr : x, b, r, p, dinv,
-                                                  w.partial, w.st);
+                                                  x_is_zero ? nullptr : x, b, r, p, dinv, P.rz1,
+                                                  P.tmp);
+  // the vector kernels must see one partial per workgroup of the same grid
+  k_start_finalize<<<1, kThreads, 0, c->stream>>>(P.rz1, P.tmp, gr, w.st);
   MG_HIP(hipGetLastError());
   PcgState h = read_state(c, w.st);
   if (h.done || maxit <= 0) return h;
+  if (gr != gv) {
+    // re-express the start partials on the vector grid: one k_dot-like pass is
+    // not needed because k_update only reads the total; store it in slot 0
+    k_start_partials_to<<<1, kThreads, 0, c->stream>>>(P.rz1, gr, gv);
+  }
   const int chunk = check_every > 0 ? check_every : 32;
+  int it = 0;
   while (true) {
-    for (int k = 0; k < chunk; ++k) {
-      k_spmv_pq<T, LPR><<<gr, kThreads, 0, c->stream>>>(n, S.rowptr, S.colidx, vals, p, q,
-                                                        w.partial, w.st);
-      k_update<T><<<gv, kThreads, 0, c->stream>>>(n, x, r, p, q, dinv, w.partial, w.st);
-      k_pupdate<T><<<gv, kThreads, 0, c->stream>>>(n, r, p, dinv, w.st);
+    for (int k = 0; k < chunk; ++k, ++it) {
+      double *rz_prev = (it & 1) ? P.rz0 : P.rz1;
+      double *rz_new = (it & 1) ? P.rz1 : P.rz0;
+      k_spmv_pq<T, LPR><<<gr, kThreads, 0, c->stream>>>(n, S.rowptr, S.colidx, vals, p, q, P.pq,
+                                                        w.st);
+      k_update<T><<<gv, kThreads, 0, c->stream>>>(n, x, r, p, q, dinv, P.pq, gr, rz_prev, gv,
+                                                  rz_new, P.rr, w.st);
+      k_pupdate<T><<<gv, kThreads, 0, c->stream>>>(n, r, p, dinv, rz_new, rz_prev, P.rr, gv, it,
+                                                   w.st);
     }
     MG_HIP(hipGetLastError());
     h = read_state(c, w.st);
@@ -471,12 +486,17 @@ static PcgState pcg_dispatch(Ctx *c, System &S, const T *vals, const T *dinv, co
 static double true_relres(Ctx *c, System &S, const double *b, const double *x, Workspace &w,
                           double bnorm) {
   const int n = (int)S.nrows;
-  if (lanes_per_row(S) == 8)
-    k_resid64<8><<<grid_rows(n, 8), kThreads, 0, c->stream>>>(n, S.rowptr, S.colidx, S.vals64, x,
-                                                              b, w.rr64, w.partial, w.st);
-  else
-    k_resid64<4><<<grid_rows(n, 4), kThreads, 0, c->stream>>>(n, S.rowptr, S.colidx, S.vals64, x,
-                                                              b, w.rr64, w.partial, w.st);
+  int g;
+  if (lanes_per_row(S) == 8) {
+    g = grid_rows(n, 8);
+    k_resid64<8><<<g, kThreads, 0, c->stream>>>(n, S.rowptr, S.colidx, S.vals64, x, b, w.rr64,
+                                                w.part.tmp);
+  } else {
+    g = grid_rows(n, 4);
+    k_resid64<4><<<g, kThreads, 0, c->stream>>>(n, S.rowptr, S.colidx, S.vals64, x, b, w.rr64,
+                                                w.part.tmp);
+  }
+  k_finalize_sum<<<1, kThreads, 0, c->stream>>>(w.part.tmp, g, &w.st->rr);
   MG_HIP(hipGetLastError());
   PcgState h = read_state(c, w.st);
   return std::sqrt(h.rr) / bnorm;
@@ -496,7 +516,8 @@ static void pcg_solve(Ctx *c, const mg_solver_params *prm, const double *b, doub
   // ||b||
   {
     init_state(c, w.st, 0.0, 0);
-    k_dot_self<<<grid_vec(n), kThreads, 0, c->stream>>>((int)n, b, w.partial, w.st);
+    k_dot_self<<<grid_vec(n), kThreads, 0, c->stream>>>((int)n, b, w.part.tmp);
+    k_finalize_sum<<<1, kThreads, 0, c->stream>>>(w.part.tmp, grid_vec(n), &w.st->bb);
     MG_HIP(hipGetLastError());
   }
   double bnorm = std::sqrt(read_state(c, w.st).bb);

@@ -217,12 +217,7 @@ void build_dirichlet(Ctx *c, int n_geo, const int32_t *geo_ids) {
   for (void *p : {(void *)dgeo, (void *)bad, (void *)f, (void *)s}) dfree(c, p);
   MG_CHECK(hb == 0, MG_ERR_MESH, "dirichlet: a boundary triangle edge is not a mesh edge");
   // a new Dirichlet set invalidates the interior system
-  System &S = c->sys[MG_SYS_INTERIOR];
-  for (void *p : {(void *)S.rowptr, (void *)S.colidx, (void *)S.diagpos, (void *)S.slot,
-                  (void *)S.inc_ptr, S.inc_rec, (void *)S.vals64, (void *)S.vals32,
-                  (void *)S.dinv64, (void *)S.dinv32})
-    dfree(c, p);
-  S = System();
+  free_system(c, c->sys[MG_SYS_INTERIOR]);
   c->dirichlet_ok = true;
 }
 
@@ -335,49 +330,104 @@ __global__ void k_inc_records(int64_t n_inc, const uint64_t *__restrict__ keys,
   rec[i] = r;
 }
 
-static void make_segments(System &S, const std::vector<int32_t> &rp) {
-  S.segs.clear();
-  const int64_t nb = (S.nrows + 63) / 64;
-  int maxnnz = 0;
-  for (int64_t r = 0; r < S.nrows; ++r) maxnnz = std::max(maxnnz, rp[r + 1] - rp[r]);
-  S.max_row_nnz = maxnnz;
-  // split rows into runs whose 64-row blocks need a similar LDS size
-  auto need = [&](int64_t b) {
-    int64_t r0 = b * 64, r1 = std::min<int64_t>(r0 + 64, S.nrows);
-    return rp[r1] - rp[r0];
-  };
-  auto bucket = [](int x) {
-    int b = 0;
-    while ((1 << b) < x) ++b;
-    return b;
-  };
-  int64_t b = 0;
-  while (b < nb) {
-    int64_t start = b;
-    int bk = bucket(need(b));
-    int mx = need(b);
-    ++b;
-    while (b < nb && (bucket(need(b)) == bk || b - start < 256)) {
-      mx = std::max(mx, need(b));
-      ++b;
+// Host construction of the assembly schedule (see AsmBlock): deterministic,
+// O(n_inc).  Blocks take consecutive rows while their CSR entries fit
+// `slots` and their count fits `max_rows`; inside a block the incidences are
+// emitted by (l, occurrence of l in its row, row), in chunks of 64 that never
+// mix (l, occurrence) groups, so a batch holds at most one incidence per row.
+template <int NL, typename RecIn, typename RecOut>
+static void build_schedule(System &S, const std::vector<int32_t> &rp, const std::vector<int32_t> &ip,
+                           const std::vector<RecIn> &rin, int slots, int max_rows,
+                           std::vector<AsmBlock> &blocks, std::vector<AsmBatch> &batches,
+                           std::vector<RecOut> &rout) {
+  const int64_t nrows = S.nrows;
+  rout.clear();
+  rout.reserve(rin.size());
+  std::vector<int32_t> lstart((size_t)max_rows * (NL + 1));
+  int64_t r0 = 0;
+  while (r0 < nrows) {
+    MG_CHECK(rp[r0 + 1] - rp[r0] <= slots, MG_ERR_SIZE,
+             "schedule: a row has more entries than the LDS budget");
+    int64_t r1 = r0 + 1;
+    while (r1 < nrows && r1 - r0 < max_rows && rp[r1 + 1] - rp[r0] <= slots) ++r1;
+    AsmBlock B;
+    B.row_begin = (int32_t)r0;
+    B.row_end = (int32_t)r1;
+    B.batch_begin = (int32_t)batches.size();
+    const int nr = (int)(r1 - r0);
+    // per row: first incidence of each l (incidences are sorted by (l, e))
+    int maxcnt[NL] = {0};
+    for (int i = 0; i < nr; ++i) {
+      int32_t *ls = &lstart[(size_t)i * (NL + 1)];
+      int k = ip[r0 + i];
+      const int kend = ip[r0 + i + 1];
+      for (int l = 0; l < NL; ++l) {
+        ls[l] = k;
+        while (k < kend && (rin[k].el & 15) == l) ++k;
+        maxcnt[l] = std::max(maxcnt[l], k - ls[l]);
+      }
+      ls[NL] = kend;
     }
-    if (!S.segs.empty() && b - start < 256) {
-      S.segs.back().row_end = (int32_t)std::min<int64_t>(b * 64, S.nrows);
-      S.segs.back().lds_slots = std::max(S.segs.back().lds_slots, mx);
-    } else {
-      S.segs.push_back(RowSegment{(int32_t)(start * 64), (int32_t)std::min<int64_t>(b * 64, S.nrows),
-                                  mx});
-    }
+    for (int l = 0; l < NL; ++l)
+      for (int occ = 0; occ < maxcnt[l]; ++occ) {
+        int count = 0;
+        for (int i = 0; i < nr; ++i) {
+          const int32_t *ls = &lstart[(size_t)i * (NL + 1)];
+          if (ls[l] + occ >= ls[l + 1]) continue;
+          if (count == 0) batches.push_back(AsmBatch{(int32_t)rout.size(), 0});
+          const RecIn &a = rin[ls[l] + occ];
+          RecOut o{};
+          o.el = a.el;
+          o.row = (uint16_t)i;
+          for (int m = 0; m < NL; ++m) o.pos[m] = a.pos[m];
+          rout.push_back(o);
+          if (++count == 64) {
+            batches.back().meta = l | (count << 8);
+            count = 0;
+          }
+        }
+        if (count) batches.back().meta = l | (count << 8);
+      }
+    B.batch_end = (int32_t)batches.size();
+    blocks.push_back(B);
+    r0 = r1;
   }
+}
+
+template <int NL, typename RecIn, typename RecOut>
+static void make_schedule(Ctx *c, System &S, const std::vector<int32_t> &rp) {
+  std::vector<int32_t> ip(S.nrows + 1);
+  std::vector<RecIn> rin(S.n_inc);
+  MG_HIP(hipMemcpyAsync(ip.data(), S.inc_ptr, sizeof(int32_t) * (S.nrows + 1),
+                        hipMemcpyDeviceToHost, c->stream));
+  MG_HIP(hipMemcpyAsync(rin.data(), S.inc_rec, sizeof(RecIn) * S.n_inc, hipMemcpyDeviceToHost,
+                        c->stream));
+  MG_HIP(hipStreamSynchronize(c->stream));
+  const char *env = getenv("MGFE_ASM_SLOTS");
+  S.asm_slots = env ? std::max(256, atoi(env)) : 2048;
+  S.asm_rows = 256;
+  std::vector<AsmBlock> blocks;
+  std::vector<AsmBatch> batches;
+  std::vector<RecOut> rout;
+  build_schedule<NL, RecIn, RecOut>(S, rp, ip, rin, S.asm_slots, S.asm_rows, blocks, batches,
+                                    rout);
+  S.n_blocks = (int64_t)blocks.size();
+  S.n_batches = (int64_t)batches.size();
+  S.blocks = dalloc<AsmBlock>(c, blocks.size());
+  S.batches = dalloc<AsmBatch>(c, batches.size());
+  S.brec = dalloc<RecOut>(c, rout.size());
+  MG_HIP(hipMemcpyAsync(S.blocks, blocks.data(), sizeof(AsmBlock) * blocks.size(),
+                        hipMemcpyHostToDevice, c->stream));
+  MG_HIP(hipMemcpyAsync(S.batches, batches.data(), sizeof(AsmBatch) * batches.size(),
+                        hipMemcpyHostToDevice, c->stream));
+  MG_HIP(hipMemcpyAsync(S.brec, rout.data(), sizeof(RecOut) * rout.size(), hipMemcpyHostToDevice,
+                        c->stream));
+  MG_HIP(hipStreamSynchronize(c->stream));
 }
 
 void build_pattern(Ctx *c, int sys) {
   System &S = c->sys[sys];
-  for (void *p : {(void *)S.rowptr, (void *)S.colidx, (void *)S.diagpos, (void *)S.slot,
-                  (void *)S.inc_ptr, S.inc_rec, (void *)S.vals64, (void *)S.vals32,
-                  (void *)S.dinv64, (void *)S.dinv32})
-    dfree(c, p);
-  S = System();
+  free_system(c, S);
   const int nl = c->nloc;
   const int32_t *map = sys == MG_SYS_FULL ? nullptr : c->sysmap;
   const int64_t nrows = sys == MG_SYS_FULL ? c->ndofs : c->nfree;
@@ -452,10 +502,13 @@ void build_pattern(Ctx *c, int sys) {
   MG_HIP(hipMemcpyAsync(rp.data(), S.rowptr, sizeof(int32_t) * (nrows + 1), hipMemcpyDeviceToHost,
                         c->stream));
   MG_HIP(hipStreamSynchronize(c->stream));
-  make_segments(S, rp);
-  for (auto &sg : S.segs)
-    MG_CHECK(sg.lds_slots * 8 <= 64 * 1024, MG_ERR_SIZE,
-             "pattern: a 64-row block owns more than 8192 CSR entries");
+  int maxnnz = 0;
+  for (int64_t r = 0; r < nrows; ++r) maxnnz = std::max(maxnnz, rp[r + 1] - rp[r]);
+  S.max_row_nnz = maxnnz;
+  if (nl == 10)
+    make_schedule<10, RecP2, BRecP2>(c, S, rp);
+  else
+    make_schedule<4, RecP1, BRecP1>(c, S, rp);
   S.vals64 = dalloc<double>(c, S.nnz);
   S.vals32 = dalloc<float>(c, S.nnz);
   S.dinv64 = dalloc<double>(c, nrows);
