@@ -1,21 +1,21 @@
 // Numeric assembly of P1/P2 tetrahedral operators into a fixed CSR pattern,
 // and the adjoint element kernel for d(loss)/d(per-element coefficient).
 //
-// Two kernels per assembly:
-//  1. k_geom (one thread per element): vertex coordinates -> J, det J (signed,
-//     as geoMapInterface::getGradientDet), J^{-1} by cofactors (tensor3d::
-//     computeInverse), and the 7 per-element factors the local matrix needs:
-//     c det (J^{-1} J^{-T})_k for the 6 symmetric pairs k, and c_mass det.
-//  2. k_asm_batched (one wavefront per schedule block of consecutive CSR rows,
-//     the block's entries staged in LDS): the block's (element, local basis l)
-//     incidences come in precomputed batches of <= 64 that share l and hit 64
-//     distinct rows.  Each lane forms row l of its element's matrix as
-//     sum_k factor_k tab[k][l][m] (tab = the quadrature of integratorOEL3d_STD
-//     contracted once on the reference element, in constant memory, read with
-//     scalar loads because l is uniform in the batch) and adds it into its
-//     row's LDS entries.  Lanes of a batch never collide and batches run in a
-//     fixed order, so the sums are deterministic without atomics.  The block's
-//     contiguous CSR value range is then written with coalesced stores.
+// One kernel per assembly launch segment (k_asm_rows): one lane owns one CSR
+// row and walks the (element, local basis l) incidences of that row, built
+// from the element -> CSR slot map.  Each element is relabelled by an even
+// vertex permutation so that the row's dof is local basis 0 (vertex) or 4
+// (edge); the quadrature rules are symmetric, so the relabelled element matrix
+// is the same matrix, and row 0 / row 4 of it is
+//   sum_k c det (J^{-1} J^{-T})_k tab[k][0 or 4][m]
+// with J, det J (signed, as geoMapInterface::getGradientDet) and the adjugate
+// computed in FP64 from the vertex coordinates, and tab = the quadrature of
+// integratorOEL3d_STD contracted once on the reference element (constant
+// memory, compile-time indices -> scalar loads).  The contributions go into the
+// row's entries staged in LDS; a row belongs to one lane and its incidences
+// are visited in a fixed order, so the sums are deterministic without atomics.
+// The workgroup then writes its contiguous CSR value range with coalesced
+// stores.
 #include <cstring>
 
 #include "mg_internal.h"
@@ -102,165 +102,207 @@ __device__ __forceinline__ GeomOut tet_geometry(const double *__restrict__ xyz, 
   return g;
 }
 
-template <typename T>
-__global__ void k_geom(int64_t nT, const int32_t *__restrict__ tets, const double *__restrict__ xyz,
-                       double ks_const, const double *__restrict__ ks_field, double km_const,
-                       const double *__restrict__ km_field, T *__restrict__ geom) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nT) return;
-  int32_t id[4];
-  const int4 t4 = *reinterpret_cast<const int4 *>(tets + 4 * e);
-  id[0] = t4.x;
-  id[1] = t4.y;
-  id[2] = t4.z;
-  id[3] = t4.w;
-  const GeomOut g = tet_geometry(xyz, id);
-  const double ks = (ks_field ? ks_field[e] : ks_const) * g.det;
-  const double km = (km_field ? km_field[e] : km_const) * g.det;
-  T out[8];
+__constant__ int c_perm[10][4];  // relabelling of the element for row basis l
+
+// 4 relabelled vertex ids of an incidence
+__device__ __forceinline__ void relabel(const int32_t *__restrict__ tets, int32_t el, int32_t id[4]) {
+  const int4 t4 = *reinterpret_cast<const int4 *>(tets + 4 * (size_t)(el >> 4));
+  const int t[4] = {t4.x, t4.y, t4.z, t4.w};
+  const int l = el & 15;
 #pragma unroll
-  for (int k = 0; k < 6; ++k) out[k] = (T)(ks * g.A[k]);
-  out[6] = (T)km;
-  out[7] = (T)0;
-  T *dst = geom + 8 * e;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) dst[k] = out[k];
+  for (int k = 0; k < 4; ++k) {
+    const int p = c_perm[l][k];
+    id[k] = p == 0 ? t[0] : p == 1 ? t[1] : p == 2 ? t[2] : t[3];
+  }
 }
 
-template <typename T>
-struct Geom8 {
-  T v[8];
+struct Coords {
+  double x[12];
+  double ks, km;
 };
 
-template <typename T>
-__device__ __forceinline__ Geom8<T> load_geom(const T *__restrict__ geom, int32_t el) {
-  Geom8<T> g;
-  const T *src = geom + 8 * (size_t)(el >> 4);
+__device__ __forceinline__ Coords load_coords(const double *__restrict__ xyz, const int32_t id[4],
+                                              int64_t e, double ks_const,
+                                              const double *__restrict__ ks_field, double km_const,
+                                              const double *__restrict__ km_field) {
+  Coords c;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) g.v[k] = src[k];
-  return g;
+  for (int v = 0; v < 4; ++v) {
+    c.x[3 * v] = xyz[3 * (size_t)id[v]];
+    c.x[3 * v + 1] = xyz[3 * (size_t)id[v] + 1];
+    c.x[3 * v + 2] = xyz[3 * (size_t)id[v] + 2];
+  }
+  c.ks = ks_field ? ks_field[e] : ks_const;
+  c.km = km_field ? km_field[e] : km_const;
+  return c;
 }
 
-// One wavefront per schedule block.  Two-deep software pipeline: while batch b
-// is contracted and added, the geometry of batch b+1 and the records of batch
-// b+2 are in flight.
-template <int NL, typename T, bool MASS, typename Rec>
-__global__ __launch_bounds__(64) void k_asm_batched(const AsmBlock *__restrict__ blocks,
-                                                    const AsmBatch *__restrict__ batches,
-                                                    const Rec *__restrict__ rec,
-                                                    const int32_t *__restrict__ rowptr,
-                                                    const T *__restrict__ geom,
-                                                    T *__restrict__ vals, int slots_cap) {
+// c det (J^{-1} J^{-T})_k for the 6 symmetric pairs, and c_mass det, of the
+// (relabelled) element: with adj = adjugate of J, det J^{-1} J^{-T} = adj adj^T / det.
+__device__ __forceinline__ void elem_factors(const Coords &c, double g[7]) {
+  double T[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) T[r][k] = c.x[3 * (k + 1) + r] - c.x[r];
+  double adj[3][3];
+  adj[0][0] = T[1][1] * T[2][2] - T[1][2] * T[2][1];
+  adj[0][1] = -(T[0][1] * T[2][2] - T[0][2] * T[2][1]);
+  adj[0][2] = T[0][1] * T[1][2] - T[0][2] * T[1][1];
+  adj[1][0] = -(T[1][0] * T[2][2] - T[1][2] * T[2][0]);
+  adj[1][1] = T[0][0] * T[2][2] - T[0][2] * T[2][0];
+  adj[1][2] = -(T[0][0] * T[1][2] - T[0][2] * T[1][0]);
+  adj[2][0] = T[1][0] * T[2][1] - T[1][1] * T[2][0];
+  adj[2][1] = -(T[0][0] * T[2][1] - T[0][1] * T[2][0]);
+  adj[2][2] = T[0][0] * T[1][1] - T[0][1] * T[1][0];
+  const double det = T[0][0] * adj[0][0] + T[0][1] * adj[1][0] + T[0][2] * adj[2][0];
+  const double s = c.ks / det;
+  const int KA[6] = {0, 1, 2, 0, 0, 1}, KB[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    g[k] = s * (adj[KA[k]][0] * adj[KB[k]][0] + adj[KA[k]][1] * adj[KB[k]][1] +
+                adj[KA[k]][2] * adj[KB[k]][2]);
+  g[6] = c.km * det;
+}
+
+// One lane per CSR row, 64 rows per workgroup, the block's entries staged in
+// LDS.  A lane walks its row's incidences in their fixed order; each element
+// is relabelled so the row is its local basis LROW, and row LROW of the element
+// matrix is sum_k g_k tab[k][LROW][m] with compile-time table indices (scalar
+// loads).  Three-stage software pipeline over the incidences: record i+2,
+// vertex ids i+1 and coordinates i are in flight while i-1 is contracted.
+template <int NL, int LROW, typename T, bool MASS, typename Rec>
+__global__ __launch_bounds__(64) void k_asm_rows(
+    int32_t row_begin, int32_t row_end, int32_t dummy, const int32_t *__restrict__ rowptr,
+    const int32_t *__restrict__ inc_ptr, const Rec *__restrict__ rec,
+    const int32_t *__restrict__ tets, const double *__restrict__ xyz, double ks_const,
+    const double *__restrict__ ks_field, double km_const, const double *__restrict__ km_field,
+    T *__restrict__ vals) {
   extern __shared__ __align__(16) char lds_raw[];
   T *acc = reinterpret_cast<T *>(lds_raw);
-  int32_t *lrp = reinterpret_cast<int32_t *>(acc + slots_cap + 1);
   const int lane = threadIdx.x;
-  const AsmBlock B = blocks[blockIdx.x];
-  const int s0 = rowptr[B.row_begin];
-  const int ns = rowptr[B.row_end] - s0;
+  const int r0 = row_begin + blockIdx.x * 64;
+  const int r1 = min(r0 + 64, row_end);
+  const int s0 = rowptr[r0];
+  const int ns = rowptr[r1] - s0;
   for (int s = lane; s < ns; s += 64) acc[s] = (T)0;
-  for (int i = lane; i <= B.row_end - B.row_begin; i += 64) lrp[i] = rowptr[B.row_begin + i] - s0;
   __syncthreads();
-  const int dummy = slots_cap;  // target of eliminated columns, never written out
-
-  int b = B.batch_begin;
-  const int bend = B.batch_end;
-  AsmBatch m0{0, 0};
-  if (b < bend) m0 = batches[b];
-  Rec r0{}, r1{};
-  int c0 = m0.meta >> 8, c1 = 0;
-  if (lane < c0) r0 = rec[m0.offset + lane];
-  Geom8<T> g0{};
-  if (lane < c0) g0 = load_geom(geom, r0.el);
-  AsmBatch m1{0, 0};
-  if (b + 1 < bend) {
-    m1 = batches[b + 1];
-    c1 = m1.meta >> 8;
-    if (lane < c1) r1 = rec[m1.offset + lane];
-  }
-  for (; b < bend; ++b) {
-    // prefetch: geometry of b+1, records of b+2
-    Geom8<T> g1{};
-    if (lane < c1) g1 = load_geom(geom, r1.el);
-    Rec r2{};
-    int c2 = 0;
-    if (b + 2 < bend) {
-      const AsmBatch m2 = batches[b + 2];
-      c2 = m2.meta >> 8;
-      if (lane < c2) r2 = rec[m2.offset + lane];
-    }
-    const int l = __builtin_amdgcn_readfirstlane(m0.meta & 0xFF);
-    if (lane < c0) {
-      const int base = lrp[r0.row];
+  const int r = r0 + lane;
+  if (r < r1) {
+    const int base = rowptr[r] - s0;
+    int i = inc_ptr[r];
+    const int iend = inc_ptr[r + 1];
+    // pipeline prologue
+    Rec ra{}, rb{}, rc{};
+    if (i < iend) ra = rec[i];
+    if (i + 1 < iend) rb = rec[i + 1];
+    int32_t ida[4] = {0, 0, 0, 0}, idb[4] = {0, 0, 0, 0};
+    if (i < iend) relabel(tets, ra.el, ida);
+    if (i + 1 < iend) relabel(tets, rb.el, idb);
+    Coords ca{};
+    if (i < iend)
+      ca = load_coords(xyz, ida, ra.el >> 4, ks_const, ks_field, km_const, km_field);
+    if (i + 2 < iend) rc = rec[i + 2];
+    for (; i < iend; ++i) {
+      // issue: coordinates of i+1, vertex ids of i+2, record of i+3
+      Coords cb{};
+      if (i + 1 < iend)
+        cb = load_coords(xyz, idb, rb.el >> 4, ks_const, ks_field, km_const, km_field);
+      int32_t idc[4] = {0, 0, 0, 0};
+      if (i + 2 < iend) relabel(tets, rc.el, idc);
+      Rec rd{};
+      if (i + 3 < iend) rd = rec[i + 3];
+      // contract incidence i
+      double g[7];
+      elem_factors(ca, g);
       T v[NL];
 #pragma unroll
       for (int m = 0; m < NL; ++m) {
-        T s = g0.v[0] * tab<T>((0 * NL + l) * NL + m);
+        T sum = (T)g[0] * tab<T>((0 * NL + LROW) * NL + m);
 #pragma unroll
-        for (int k = 1; k < 6; ++k) s += g0.v[k] * tab<T>((k * NL + l) * NL + m);
-        if (MASS) s += g0.v[6] * tab<T>((6 * NL + l) * NL + m);
-        v[m] = s;
+        for (int k = 1; k < 6; ++k) sum += (T)g[k] * tab<T>((k * NL + LROW) * NL + m);
+        if (MASS) sum += (T)g[6] * tab<T>((6 * NL + LROW) * NL + m);
+        v[m] = sum;
       }
       int idx[NL];
 #pragma unroll
-      for (int m = 0; m < NL; ++m) idx[m] = r0.pos[m] == 0xFF ? dummy : base + r0.pos[m];
+      for (int m = 0; m < NL; ++m) idx[m] = ra.pos[m] == 0xFF ? dummy : base + ra.pos[m];
       T a[NL];
 #pragma unroll
       for (int m = 0; m < NL; ++m) a[m] = acc[idx[m]];
 #pragma unroll
       for (int m = 0; m < NL; ++m) acc[idx[m]] = a[m] + v[m];
+      // rotate
+      ra = rb;
+      rb = rc;
+      rc = rd;
+      ca = cb;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        idb[k] = idc[k];
+      }
     }
-    // rotate the pipeline
-    m0 = m1;
-    c0 = c1;
-    r0 = r1;
-    g0 = g1;
-    c1 = c2;
-    r1 = r2;
-    if (b + 2 < bend) m1 = batches[b + 2];
   }
   __syncthreads();
   for (int s = lane; s < ns; s += 64) vals[s0 + s] = acc[s];
 }
 
 template <int NL, typename T, typename Rec>
-static void launch_rows(Ctx *c, System &S, bool mass, const T *geom, T *vals) {
-  const size_t lds = sizeof(T) * (size_t)(S.asm_slots + 1) + sizeof(int32_t) * (S.asm_rows + 1);
-  const int nblk = (int)S.n_blocks;
-  if (mass)
-    k_asm_batched<NL, T, true, Rec><<<nblk, 64, lds, c->stream>>>(
-        S.blocks, S.batches, (const Rec *)S.brec, S.rowptr, geom, vals, S.asm_slots);
-  else
-    k_asm_batched<NL, T, false, Rec><<<nblk, 64, lds, c->stream>>>(
-        S.blocks, S.batches, (const Rec *)S.brec, S.rowptr, geom, vals, S.asm_slots);
-  MG_HIP(hipGetLastError());
+static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals) {
+  const bool stiff = op->terms & MG_TERM_STIFF, mass = op->terms & MG_TERM_MASS;
+  const double ksc = stiff ? op->stiff_const : 0.0, kmc = mass ? op->mass_const : 0.0;
+  const double *ksf = (stiff && op->stiff_coeff == MG_COEFF_P0) ? c->field[MG_FIELD_STIFF] : nullptr;
+  const double *kmf = (mass && op->mass_coeff == MG_COEFF_P0) ? c->field[MG_FIELD_MASS] : nullptr;
+  for (const RowSegment &sg : S.segs) {
+    const int nblk = (sg.row_end - sg.row_begin + 63) / 64;
+    const size_t lds = sizeof(T) * (size_t)(sg.lds_slots + 1);
+    const Rec *rec = (const Rec *)S.inc_rec;
+#define MG_LAUNCH(LR, MS)                                                                     \
+  k_asm_rows<NL, LR, T, MS, Rec><<<nblk, 64, lds, c->stream>>>(                               \
+      sg.row_begin, sg.row_end, sg.lds_slots, S.rowptr, S.inc_ptr, rec, c->tets, c->xyz, ksc, \
+      ksf, kmc, kmf, vals)
+    if (sg.lrow == 0) {
+      if (mass)
+        MG_LAUNCH(0, true);
+      else
+        MG_LAUNCH(0, false);
+    } else {
+      if (NL == 10) {
+        if (mass)
+          MG_LAUNCH(4, true);
+        else
+          MG_LAUNCH(4, false);
+      }
+    }
+#undef MG_LAUNCH
+    MG_HIP(hipGetLastError());
+  }
 }
 
 static bool rule_ok(int r) { return r == 1 || r == 4 || r == 11 || r == 15; }
+
+static void upload_perm(Ctx *c) {
+  static bool done[64] = {false};
+  if (c->device < 64 && done[c->device]) return;
+  int perm[10][4], sigma[10][10];
+  relabel_tables(perm, sigma);
+  MG_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_perm), perm, sizeof(perm), 0, hipMemcpyHostToDevice,
+                                c->stream));
+  MG_HIP(hipStreamSynchronize(c->stream));
+  if (c->device < 64) done[c->device] = true;
+}
 
 template <typename T>
 static void assemble_t(Ctx *c, int sys, const mg_operator *op, int prec) {
   System &S = c->sys[sys];
   const bool stiff = op->terms & MG_TERM_STIFF, mass = op->terms & MG_TERM_MASS;
   upload_tables(c, c->order, stiff ? op->stiff_rule : 0, mass ? op->mass_rule : 0, prec);
-  const size_t gb = sizeof(T) * 8 * (size_t)c->nT;
-  if (c->geom_bytes < gb) {
-    dfree(c, c->geom);
-    c->geom = dalloc<char>(c, gb);
-    c->geom_bytes = gb;
-  }
-  T *geom = (T *)c->geom;
-  const double ksc = stiff ? op->stiff_const : 0.0, kmc = mass ? op->mass_const : 0.0;
-  const double *ksf = (stiff && op->stiff_coeff == MG_COEFF_P0) ? c->field[MG_FIELD_STIFF] : nullptr;
-  const double *kmf = (mass && op->mass_coeff == MG_COEFF_P0) ? c->field[MG_FIELD_MASS] : nullptr;
-  k_geom<T><<<blocks_for(c->nT, 256), 256, 0, c->stream>>>(c->nT, c->tets, c->xyz, ksc, ksf, kmc,
-                                                         kmf, geom);
-  MG_HIP(hipGetLastError());
+  upload_perm(c);
   T *vals = prec == MG_PREC_F64 ? (T *)S.vals64 : (T *)S.vals32;
   if (c->nloc == 10)
-    launch_rows<10, T, BRecP2>(c, S, mass, geom, vals);
+    launch_rows<10, T, RecP2>(c, S, op, vals);
   else
-    launch_rows<4, T, BRecP1>(c, S, mass, geom, vals);
+    launch_rows<4, T, RecP1>(c, S, op, vals);
 }
 
 // ---- adjoint element kernel ---------------------------------------------------

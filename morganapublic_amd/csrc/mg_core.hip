@@ -1,5 +1,6 @@
 // Context, mesh upload/validation, P0 fields and the host-side reference
 // tables (quadrature + reference basis) of libmgfe.so.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -21,8 +22,7 @@ void dfree(Ctx *c, void *p) {
 void free_system(Ctx *c, System &S) {
   for (void *p : {(void *)S.rowptr, (void *)S.colidx, (void *)S.diagpos, (void *)S.slot,
                   (void *)S.inc_ptr, S.inc_rec, (void *)S.vals64, (void *)S.vals32,
-                  (void *)S.dinv64, (void *)S.dinv32, (void *)S.blocks, (void *)S.batches,
-                  S.brec})
+                  (void *)S.dinv64, (void *)S.dinv32})
     dfree(c, p);
   S = System();
 }
@@ -108,7 +108,41 @@ int quad_rule(int npts, double *pts, double *w) {
 // P1: lambda_i.  P2 (order of fePr3d<2>, fePr3d.hpp:553-592): vertices
 // lambda_i (2 lambda_i - 1), then edges 12, 23, 31, 14, 24, 34 (geoShapes.cpp
 // edgeToPoint) as 4 lambda_a lambda_b.
-static const int kEdge[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+const int kEdgeV[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+
+void relabel_tables(int perm[10][4], int sigma[10][10]) {
+  int p[4] = {0, 1, 2, 3};
+  for (int l = 0; l < 10; ++l) {
+    // first even permutation (lexicographic order) with the required property
+    int q[4] = {0, 1, 2, 3};
+    bool found = false;
+    do {
+      int inv = 0;
+      for (int i = 0; i < 4; ++i)
+        for (int j = i + 1; j < 4; ++j) inv += q[i] > q[j];
+      if (inv & 1) continue;
+      const bool ok = l < 4 ? q[0] == l
+                            : ((q[0] == kEdgeV[l - 4][0] && q[1] == kEdgeV[l - 4][1]) ||
+                               (q[0] == kEdgeV[l - 4][1] && q[1] == kEdgeV[l - 4][0]));
+      if (ok) {
+        for (int i = 0; i < 4; ++i) p[i] = q[i];
+        found = true;
+        break;
+      }
+    } while (std::next_permutation(q, q + 4));
+    (void)found;
+    for (int i = 0; i < 4; ++i) perm[l][i] = p[i];
+    for (int m = 0; m < 4; ++m) sigma[l][m] = p[m];
+    for (int j = 0; j < 6; ++j) {
+      const int a = p[kEdgeV[j][0]], b = p[kEdgeV[j][1]];
+      int old = -1;
+      for (int k = 0; k < 6; ++k)
+        if ((kEdgeV[k][0] == a && kEdgeV[k][1] == b) || (kEdgeV[k][0] == b && kEdgeV[k][1] == a))
+          old = k;
+      sigma[l][4 + j] = 4 + old;
+    }
+  }
+}
 
 void ref_basis(int order, const double *y, double *val, double *grad) {
   const double L[4] = {1.0 - y[0] - y[1] - y[2], y[0], y[1], y[2]};
@@ -128,7 +162,7 @@ void ref_basis(int order, const double *y, double *val, double *grad) {
       for (int d = 0; d < 3; ++d) grad[d * n + i] = (4.0 * L[i] - 1.0) * dL[i][d];
   }
   for (int j = 0; j < 6; ++j) {
-    const int a = kEdge[j][0], b = kEdge[j][1];
+    const int a = kEdgeV[j][0], b = kEdgeV[j][1];
     if (val) val[4 + j] = 4.0 * L[a] * L[b];
     if (grad)
       for (int d = 0; d < 3; ++d) grad[d * n + 4 + j] = 4.0 * (L[a] * dL[b][d] + L[b] * dL[a][d]);

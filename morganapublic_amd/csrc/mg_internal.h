@@ -12,10 +12,11 @@
 
 namespace mg {
 
-// Incidence record: element e sees row dof as its local basis l; pos[m] is
-// the position, inside that row, of the column dof of local basis m (0xFF when
-// the column is eliminated).  Built once per pattern, then reordered into the
-// assembly schedule below.
+// Incidence record: element e sees the row dof as its local basis l.  The
+// element is relabelled by the even vertex permutation perm[l] (relabel_tables)
+// that moves that basis to local index 0 (vertex) or 4 (edge); pos[m'] is the
+// position, inside the row, of the column dof of relabelled basis m' (0xFF when
+// the column is eliminated).
 struct alignas(16) RecP2 {
   int32_t el;  // e << 4 | l
   uint8_t pos[10];
@@ -26,28 +27,11 @@ struct alignas(8) RecP1 {
   uint8_t pos[4];
 };
 
-// Assembly schedule.  Rows are cut into blocks whose CSR entries fit the LDS
-// budget; the incidences of a block are packed into batches of at most 64 that
-// share the local basis l (so the element tensors are read with scalar loads)
-// and come from 64 distinct rows (so their LDS updates never collide).  One
-// wavefront runs one block, batch after batch, in a fixed order.
-struct AsmBlock {
-  int32_t row_begin, row_end, batch_begin, batch_end;
-};
-struct AsmBatch {
-  int32_t offset;  // first record of the batch
-  int32_t meta;    // l | count << 8
-};
-struct alignas(16) BRecP2 {
-  int32_t el;    // e << 4 | l
-  uint16_t row;  // row index inside the block
-  uint8_t pos[10];
-};
-struct alignas(4) BRecP1 {
-  int32_t el;
-  uint16_t row;
-  uint8_t pos[4];
-  uint8_t pad[2];
+// Rows of one assembly launch: 64-row blocks, each staging its CSR entries in
+// LDS; lrow is the local basis every incidence of these rows is relabelled to
+// (0 for vertex-dof rows, 4 for P2 edge-dof rows).
+struct RowSegment {
+  int32_t row_begin, row_end, lds_slots, lrow;
 };
 
 struct System {
@@ -60,12 +44,7 @@ struct System {
   int32_t *slot = nullptr;     // nT * nloc * nloc element -> CSR slot map
   int32_t *inc_ptr = nullptr;  // nrows + 1 (row -> incidences)
   void *inc_rec = nullptr;     // n_inc records (RecP1 or RecP2), sorted by (row, l, e)
-  // assembly schedule
-  int32_t asm_slots = 0, asm_rows = 0;  // LDS entries / rows per block (upper bounds)
-  int64_t n_blocks = 0, n_batches = 0;
-  AsmBlock *blocks = nullptr;
-  AsmBatch *batches = nullptr;
-  void *brec = nullptr;  // n_inc BRecP1 / BRecP2 in batch order
+  std::vector<RowSegment> segs;
   double *vals64 = nullptr;
   float *vals32 = nullptr;
   bool has64 = false, has32 = false;
@@ -100,7 +79,7 @@ struct Ctx {
   // Dirichlet
   uint8_t *fixed = nullptr;   // ndofs
   int32_t *sysmap = nullptr;  // ndofs -> interior index or -1
-  int64_t nfree = 0;
+  int64_t nfree = 0, nfree_vertex = 0;
   bool dirichlet_ok = false;
 
   System sys[2];
@@ -163,6 +142,13 @@ void build_dirichlet(Ctx *c, int n_geo, const int32_t *geo_ids);
 void build_pattern(Ctx *c, int sys);
 void ensure_dinv(Ctx *c, int sys, int prec);
 void spmv_launch(Ctx *c, int sys, int prec, const void *x, void *y);
+
+// Even vertex permutations perm[l] (new local vertex -> old) moving local basis
+// l to 0 (l < 4) or local edge l-4 to edge 0 = (0,1); sigma[l][m'] = old local
+// basis of relabelled basis m'.  The quadrature rules are invariant under
+// vertex permutations, so the relabelled element matrix is the same matrix.
+void relabel_tables(int perm[10][4], int sigma[10][10]);
+extern const int kEdgeV[6][2];
 
 // host-side reference tables (restating the reference basis and quadrature)
 int quad_rule(int npts, double *pts, double *w);

@@ -56,13 +56,30 @@ __device__ __forceinline__ double total(const double *__restrict__ partial, int 
   return block_sum_all(v, sh);
 }
 
+// Dot product of one CSR row with x by LPR lanes; each round issues U
+// independent (column, value) loads per lane before the U gathers of x, so a
+// lane keeps U loads in flight.
 template <typename T, int LPR>
 __device__ __forceinline__ T row_dot(int row, int lane, const int32_t *__restrict__ rowptr,
                                      const int32_t *__restrict__ colidx, const T *__restrict__ vals,
                                      const T *__restrict__ x) {
+  constexpr int U = 4;
   const int k0 = rowptr[row], k1 = rowptr[row + 1];
   T s = (T)0;
-  for (int k = k0 + lane; k < k1; k += LPR) s += vals[k] * x[colidx[k]];
+  for (int kb = k0 + lane; kb < k1; kb += LPR * U) {
+    int col[U];
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = kb + u * LPR;
+      const bool ok = k < k1;
+      col[u] = ok ? colidx[k] : 0;
+      v[u] = ok ? vals[k] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (kb + u * LPR < k1) s += v[u] * x[col[u]];
+  }
 #pragma unroll
   for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
   return s;
@@ -565,12 +582,16 @@ static void pcg_solve(Ctx *c, const mg_solver_params *prm, const double *b, doub
       }
       if (outer == max_outer || iters >= maxit) break;
       const double rn = relres * bnorm;
+      // the inner solve never goes below inner_tol, nor further than this step
+      // needs to reach tol (with a factor 1/4 of margin)
+      const double need = 0.25 * tol / relres;
+      const double step_tol = need > itol ? (need < 0.5 ? need : 0.5) : itol;
       k_to_f32<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, w.rr64, 1.0 / rn, w.b32);
       MG_HIP(hipGetLastError());
       // inner: A32 d = r32 / ||r||, ||r32|| = 1
       PcgState h = pcg_dispatch<float>(c, S, S.vals32, S.dinv32, w.b32, w.x32, w.r32, w.p32,
-                                       w.q32, true, itol * itol, maxit - iters, prm->check_every,
-                                       w);
+                                       w.q32, true, step_tol * step_tol, maxit - iters,
+                                       prm->check_every, w);
       iters += h.iters;
       if (h.status == MG_ERR_BREAKDOWN) {
         status = MG_ERR_BREAKDOWN;

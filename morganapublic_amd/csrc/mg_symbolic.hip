@@ -14,6 +14,7 @@
 namespace mg {
 
 __constant__ int c_edge_v[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+__constant__ int c_sigma[10][10];  // relabelled basis -> original basis, per row basis l
 
 static int bitlen(uint64_t x) {
   int b = 0;
@@ -213,6 +214,7 @@ void build_dirichlet(Ctx *c, int n_geo, const int32_t *geo_ids) {
   k_sysmap<<<blocks_for(N, 256), 256, 0, c->stream>>>(N, c->fixed, s, c->sysmap);
   MG_HIP(hipGetLastError());
   c->nfree = read_back(c, s + N);
+  c->nfree_vertex = read_back(c, s + c->nV);
   int hb = read_back(c, bad);
   for (void *p : {(void *)dgeo, (void *)bad, (void *)f, (void *)s}) dfree(c, p);
   MG_CHECK(hb == 0, MG_ERR_MESH, "dirichlet: a boundary triangle edge is not a mesh edge");
@@ -319,7 +321,8 @@ __global__ void k_inc_records(int64_t n_inc, const uint64_t *__restrict__ keys,
   Rec r;
   r.el = v;
   for (int m = 0; m < NL; ++m) {
-    int32_t s = slot[e * NL * NL + m * NL + l];  // row basis l, column basis m
+    // column of relabelled basis m = original basis c_sigma[l][m]; row basis l
+    int32_t s = slot[e * NL * NL + c_sigma[l][m] * NL + l];
     int p = s < 0 ? 0xFF : s - base;
     if (s >= 0 && (p < 0 || p >= 0xFF)) {
       atomicAdd(bad, 1);
@@ -330,99 +333,19 @@ __global__ void k_inc_records(int64_t n_inc, const uint64_t *__restrict__ keys,
   rec[i] = r;
 }
 
-// Host construction of the assembly schedule (see AsmBlock): deterministic,
-// O(n_inc).  Blocks take consecutive rows while their CSR entries fit
-// `slots` and their count fits `max_rows`; inside a block the incidences are
-// emitted by (l, occurrence of l in its row, row), in chunks of 64 that never
-// mix (l, occurrence) groups, so a batch holds at most one incidence per row.
-template <int NL, typename RecIn, typename RecOut>
-static void build_schedule(System &S, const std::vector<int32_t> &rp, const std::vector<int32_t> &ip,
-                           const std::vector<RecIn> &rin, int slots, int max_rows,
-                           std::vector<AsmBlock> &blocks, std::vector<AsmBatch> &batches,
-                           std::vector<RecOut> &rout) {
-  const int64_t nrows = S.nrows;
-  rout.clear();
-  rout.reserve(rin.size());
-  std::vector<int32_t> lstart((size_t)max_rows * (NL + 1));
-  int64_t r0 = 0;
-  while (r0 < nrows) {
-    MG_CHECK(rp[r0 + 1] - rp[r0] <= slots, MG_ERR_SIZE,
-             "schedule: a row has more entries than the LDS budget");
-    int64_t r1 = r0 + 1;
-    while (r1 < nrows && r1 - r0 < max_rows && rp[r1 + 1] - rp[r0] <= slots) ++r1;
-    AsmBlock B;
-    B.row_begin = (int32_t)r0;
-    B.row_end = (int32_t)r1;
-    B.batch_begin = (int32_t)batches.size();
-    const int nr = (int)(r1 - r0);
-    // per row: first incidence of each l (incidences are sorted by (l, e))
-    int maxcnt[NL] = {0};
-    for (int i = 0; i < nr; ++i) {
-      int32_t *ls = &lstart[(size_t)i * (NL + 1)];
-      int k = ip[r0 + i];
-      const int kend = ip[r0 + i + 1];
-      for (int l = 0; l < NL; ++l) {
-        ls[l] = k;
-        while (k < kend && (rin[k].el & 15) == l) ++k;
-        maxcnt[l] = std::max(maxcnt[l], k - ls[l]);
-      }
-      ls[NL] = kend;
-    }
-    for (int l = 0; l < NL; ++l)
-      for (int occ = 0; occ < maxcnt[l]; ++occ) {
-        int count = 0;
-        for (int i = 0; i < nr; ++i) {
-          const int32_t *ls = &lstart[(size_t)i * (NL + 1)];
-          if (ls[l] + occ >= ls[l + 1]) continue;
-          if (count == 0) batches.push_back(AsmBatch{(int32_t)rout.size(), 0});
-          const RecIn &a = rin[ls[l] + occ];
-          RecOut o{};
-          o.el = a.el;
-          o.row = (uint16_t)i;
-          for (int m = 0; m < NL; ++m) o.pos[m] = a.pos[m];
-          rout.push_back(o);
-          if (++count == 64) {
-            batches.back().meta = l | (count << 8);
-            count = 0;
-          }
-        }
-        if (count) batches.back().meta = l | (count << 8);
-      }
-    B.batch_end = (int32_t)batches.size();
-    blocks.push_back(B);
-    r0 = r1;
-  }
-}
-
-template <int NL, typename RecIn, typename RecOut>
-static void make_schedule(Ctx *c, System &S, const std::vector<int32_t> &rp) {
-  std::vector<int32_t> ip(S.nrows + 1);
-  std::vector<RecIn> rin(S.n_inc);
-  MG_HIP(hipMemcpyAsync(ip.data(), S.inc_ptr, sizeof(int32_t) * (S.nrows + 1),
-                        hipMemcpyDeviceToHost, c->stream));
-  MG_HIP(hipMemcpyAsync(rin.data(), S.inc_rec, sizeof(RecIn) * S.n_inc, hipMemcpyDeviceToHost,
-                        c->stream));
-  MG_HIP(hipStreamSynchronize(c->stream));
-  const char *env = getenv("MGFE_ASM_SLOTS");
-  S.asm_slots = env ? std::max(256, atoi(env)) : 2048;
-  S.asm_rows = 256;
-  std::vector<AsmBlock> blocks;
-  std::vector<AsmBatch> batches;
-  std::vector<RecOut> rout;
-  build_schedule<NL, RecIn, RecOut>(S, rp, ip, rin, S.asm_slots, S.asm_rows, blocks, batches,
-                                    rout);
-  S.n_blocks = (int64_t)blocks.size();
-  S.n_batches = (int64_t)batches.size();
-  S.blocks = dalloc<AsmBlock>(c, blocks.size());
-  S.batches = dalloc<AsmBatch>(c, batches.size());
-  S.brec = dalloc<RecOut>(c, rout.size());
-  MG_HIP(hipMemcpyAsync(S.blocks, blocks.data(), sizeof(AsmBlock) * blocks.size(),
-                        hipMemcpyHostToDevice, c->stream));
-  MG_HIP(hipMemcpyAsync(S.batches, batches.data(), sizeof(AsmBatch) * batches.size(),
-                        hipMemcpyHostToDevice, c->stream));
-  MG_HIP(hipMemcpyAsync(S.brec, rout.data(), sizeof(RecOut) * rout.size(), hipMemcpyHostToDevice,
-                        c->stream));
-  MG_HIP(hipStreamSynchronize(c->stream));
+// Assembly launches: the vertex-dof rows (relabelled local basis 0) and the
+// edge-dof rows (basis 4) of a P2 system go to separate launches; each launch
+// uses 64-row blocks and the LDS size of its largest block.
+static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvrows) {
+  S.segs.clear();
+  auto add = [&](int64_t a, int64_t b, int lrow) {
+    if (b <= a) return;
+    int mx = 0;
+    for (int64_t r = a; r < b; r += 64) mx = std::max(mx, rp[std::min<int64_t>(r + 64, b)] - rp[r]);
+    S.segs.push_back(RowSegment{(int32_t)a, (int32_t)b, mx, lrow});
+  };
+  add(0, nvrows, 0);
+  add(nvrows, S.nrows, 4);
 }
 
 void build_pattern(Ctx *c, int sys) {
@@ -483,6 +406,13 @@ void build_pattern(Ctx *c, int sys) {
   S.n_inc = read_back(c, S.inc_ptr + nrows);
   int *bad = dalloc<int>(c, 1);
   MG_HIP(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+  {
+    int perm[10][4], sigma[10][10];
+    relabel_tables(perm, sigma);
+    MG_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sigma), sigma, sizeof(sigma), 0,
+                                  hipMemcpyHostToDevice, c->stream));
+    MG_HIP(hipStreamSynchronize(c->stream));
+  }
   if (nl == 10) {
     S.inc_rec = dalloc<RecP2>(c, S.n_inc);
     k_inc_records<10, RecP2><<<blocks_for(S.n_inc, 256), 256, 0, c->stream>>>(
@@ -505,10 +435,12 @@ void build_pattern(Ctx *c, int sys) {
   int maxnnz = 0;
   for (int64_t r = 0; r < nrows; ++r) maxnnz = std::max(maxnnz, rp[r + 1] - rp[r]);
   S.max_row_nnz = maxnnz;
-  if (nl == 10)
-    make_schedule<10, RecP2, BRecP2>(c, S, rp);
-  else
-    make_schedule<4, RecP1, BRecP1>(c, S, rp);
+  const int64_t nvrows =
+      nl == 4 ? nrows : (sys == MG_SYS_FULL ? c->nV : c->nfree_vertex);
+  make_segments(S, rp, nvrows);
+  for (const RowSegment &sg : S.segs)
+    MG_CHECK(sg.lds_slots * 8 <= 64 * 1024 - 64, MG_ERR_SIZE,
+             "pattern: a 64-row block owns more than 8184 CSR entries");
   S.vals64 = dalloc<double>(c, S.nnz);
   S.vals32 = dalloc<float>(c, S.nnz);
   S.dinv64 = dalloc<double>(c, nrows);
