@@ -102,18 +102,80 @@ __device__ __forceinline__ GeomOut tet_geometry(const double *__restrict__ xyz, 
   return g;
 }
 
-__constant__ int c_perm[10][4];  // relabelling of the element for row basis l
+// The 10 relabelling permutations packed 2 bits per entry, 8 bits per l:
+// byte l of (perm_hi:perm_lo) = perm[l][0] | perm[l][1] << 2 | ... (kernel
+// arguments, decoded with ALU ops so no memory access depends on l).
+struct PermPack {
+  uint64_t lo;
+  uint32_t hi;
+};
 
-// 4 relabelled vertex ids of an incidence
-__device__ __forceinline__ void relabel(const int32_t *__restrict__ tets, int32_t el, int32_t id[4]) {
-  const int4 t4 = *reinterpret_cast<const int4 *>(tets + 4 * (size_t)(el >> 4));
-  const int t[4] = {t4.x, t4.y, t4.z, t4.w};
+static PermPack pack_perm() {
+  int perm[10][4], sigma[10][10];
+  relabel_tables(perm, sigma);
+  PermPack p{0, 0};
+  for (int l = 0; l < 10; ++l) {
+    uint64_t b = perm[l][0] | perm[l][1] << 2 | perm[l][2] << 4 | perm[l][3] << 6;
+    if (l < 8)
+      p.lo |= b << (8 * l);
+    else
+      p.hi |= (uint32_t)(b << (8 * (l - 8)));
+  }
+  return p;
+}
+
+__device__ __forceinline__ int4 load_tet(const int32_t *__restrict__ tets, int32_t el) {
+  return *reinterpret_cast<const int4 *>(tets + 4 * (size_t)(el >> 4));
+}
+
+// the 4 vertex ids of the element relabelled for row basis l = el & 15
+__device__ __forceinline__ void relabel(int4 t4, int32_t el, PermPack pp, int32_t id[4]) {
   const int l = el & 15;
+  const uint32_t byte = l < 8 ? (uint32_t)(pp.lo >> (8 * l)) : (pp.hi >> (8 * (l - 8)));
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const int p = c_perm[l][k];
-    id[k] = p == 0 ? t[0] : p == 1 ? t[1] : p == 2 ? t[2] : t[3];
+    const uint32_t p = (byte >> (2 * k)) & 3u;
+    const int a = (p & 1u) ? t4.y : t4.x;
+    const int b = (p & 1u) ? t4.w : t4.z;
+    id[k] = (p & 2u) ? b : a;
   }
+}
+
+// An incidence record read as whole dwords: el, then the pos bytes.
+template <int NL>
+struct RecWords;
+template <>
+struct RecWords<10> {
+  int32_t el;
+  uint32_t w[3];
+  __device__ __forceinline__ int pos(int m) const { return (w[m >> 2] >> (8 * (m & 3))) & 0xFF; }
+};
+template <>
+struct RecWords<4> {
+  int32_t el;
+  uint32_t w[1];
+  __device__ __forceinline__ int pos(int m) const { return (w[0] >> (8 * m)) & 0xFF; }
+};
+
+template <int NL>
+__device__ __forceinline__ RecWords<NL> load_rec(const void *__restrict__ rec, int j);
+template <>
+__device__ __forceinline__ RecWords<10> load_rec<10>(const void *__restrict__ rec, int j) {
+  const int4 q = reinterpret_cast<const int4 *>(rec)[j];  // RecP2: el, pos[10], pad[2]
+  RecWords<10> r;
+  r.el = q.x;
+  r.w[0] = (uint32_t)q.y;
+  r.w[1] = (uint32_t)q.z;
+  r.w[2] = (uint32_t)q.w;
+  return r;
+}
+template <>
+__device__ __forceinline__ RecWords<4> load_rec<4>(const void *__restrict__ rec, int j) {
+  const int2 q = reinterpret_cast<const int2 *>(rec)[j];  // RecP1: el, pos[4]
+  RecWords<4> r;
+  r.el = q.x;
+  r.w[0] = (uint32_t)q.y;
+  return r;
 }
 
 struct Coords {
@@ -169,11 +231,13 @@ __device__ __forceinline__ void elem_factors(const Coords &c, double g[7]) {
 // LDS.  A lane walks its row's incidences in their fixed order; each element
 // is relabelled so the row is its local basis LROW, and row LROW of the element
 // matrix is sum_k g_k tab[k][LROW][m] with compile-time table indices (scalar
-// loads).  Three-stage software pipeline over the incidences: record i+2,
-// vertex ids i+1 and coordinates i are in flight while i-1 is contracted.
+// loads).  Software pipeline over the incidences: while incidence j is
+// contracted, the coordinates of j+1, the vertex ids of j+2 and the record of
+// j+3 are in flight.
 template <int NL, int LROW, typename T, bool MASS, typename Rec>
 __global__ __launch_bounds__(64) void k_asm_rows(
-    int32_t row_begin, int32_t row_end, int32_t dummy, const int32_t *__restrict__ rowptr,
+    int32_t row_begin, int32_t row_end, int32_t dummy, PermPack pp,
+    const int32_t *__restrict__ rowptr,
     const int32_t *__restrict__ inc_ptr, const Rec *__restrict__ rec,
     const int32_t *__restrict__ tets, const double *__restrict__ xyz, double ks_const,
     const double *__restrict__ ks_field, double km_const, const double *__restrict__ km_field,
@@ -192,29 +256,32 @@ __global__ __launch_bounds__(64) void k_asm_rows(
     const int base = rowptr[r] - s0;
     int i = inc_ptr[r];
     const int iend = inc_ptr[r + 1];
-    // pipeline prologue
-    Rec ra{}, rb{}, rc{};
-    if (i < iend) ra = rec[i];
-    if (i + 1 < iend) rb = rec[i + 1];
-    int32_t ida[4] = {0, 0, 0, 0}, idb[4] = {0, 0, 0, 0};
-    if (i < iend) relabel(tets, ra.el, ida);
-    if (i + 1 < iend) relabel(tets, rb.el, idb);
-    Coords ca{};
-    if (i < iend)
-      ca = load_coords(xyz, ida, ra.el >> 4, ks_const, ks_field, km_const, km_field);
-    if (i + 2 < iend) rc = rec[i + 2];
-    for (; i < iend; ++i) {
-      // issue: coordinates of i+1, vertex ids of i+2, record of i+3
-      Coords cb{};
-      if (i + 1 < iend)
-        cb = load_coords(xyz, idb, rb.el >> 4, ks_const, ks_field, km_const, km_field);
-      int32_t idc[4] = {0, 0, 0, 0};
-      if (i + 2 < iend) relabel(tets, rc.el, idc);
-      Rec rd{};
-      if (i + 3 < iend) rd = rec[i + 3];
-      // contract incidence i
+    // Three slots rotate through the stages (record -> vertex ids ->
+    // coordinates -> contraction); each step issues the next stage of the two
+    // following incidences and the record three ahead, then contracts the
+    // current one.  Unrolled by three so every load lands in the slot that
+    // consumes it and no loaded register is copied; the body has no branch:
+    // indices past the row's last incidence are clamped (loads stay in
+    // bounds) and their contribution goes to the dummy slot with value 0.
+    const int i0 = i;
+    const int last = iend > i0 ? iend - 1 : 0;  // a row without incidences reads record 0
+    auto at = [&](int j) { return j < iend ? j : last; };
+    struct Slot {
+      RecWords<NL> r;
+      int4 t;
+      Coords c;
+    };
+    Slot S0, S1, S2;
+    auto step = [&](Slot &A, Slot &B, Slot &C, int j) {
+      {
+        int32_t id[4];
+        relabel(B.t, B.r.el, pp, id);
+        B.c = load_coords(xyz, id, B.r.el >> 4, ks_const, ks_field, km_const, km_field);
+      }
+      C.t = load_tet(tets, C.r.el);
+      const bool valid = j < iend;
       double g[7];
-      elem_factors(ca, g);
+      elem_factors(A.c, g);
       T v[NL];
 #pragma unroll
       for (int m = 0; m < NL; ++m) {
@@ -222,25 +289,34 @@ __global__ __launch_bounds__(64) void k_asm_rows(
 #pragma unroll
         for (int k = 1; k < 6; ++k) sum += (T)g[k] * tab<T>((k * NL + LROW) * NL + m);
         if (MASS) sum += (T)g[6] * tab<T>((6 * NL + LROW) * NL + m);
-        v[m] = sum;
+        v[m] = valid ? sum : (T)0;
       }
       int idx[NL];
 #pragma unroll
-      for (int m = 0; m < NL; ++m) idx[m] = ra.pos[m] == 0xFF ? dummy : base + ra.pos[m];
+      for (int m = 0; m < NL; ++m) {
+        const int pm = A.r.pos(m);
+        idx[m] = (pm == 0xFF || !valid) ? dummy : base + pm;
+      }
       T a[NL];
 #pragma unroll
       for (int m = 0; m < NL; ++m) a[m] = acc[idx[m]];
 #pragma unroll
       for (int m = 0; m < NL; ++m) acc[idx[m]] = a[m] + v[m];
-      // rotate
-      ra = rb;
-      rb = rc;
-      rc = rd;
-      ca = cb;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        idb[k] = idc[k];
-      }
+      A.r = load_rec<NL>(rec, at(j + 3));
+    };
+    S0.r = load_rec<NL>(rec, at(i));
+    {
+      int32_t id[4];
+      relabel(load_tet(tets, S0.r.el), S0.r.el, pp, id);
+      S0.c = load_coords(xyz, id, S0.r.el >> 4, ks_const, ks_field, km_const, km_field);
+    }
+    S1.r = load_rec<NL>(rec, at(i + 1));
+    S1.t = load_tet(tets, S1.r.el);
+    S2.r = load_rec<NL>(rec, at(i + 2));
+    for (; i < iend; i += 3) {
+      step(S0, S1, S2, i);
+      step(S1, S2, S0, i + 1);
+      step(S2, S0, S1, i + 2);
     }
   }
   __syncthreads();
@@ -253,13 +329,15 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals) {
   const double ksc = stiff ? op->stiff_const : 0.0, kmc = mass ? op->mass_const : 0.0;
   const double *ksf = (stiff && op->stiff_coeff == MG_COEFF_P0) ? c->field[MG_FIELD_STIFF] : nullptr;
   const double *kmf = (mass && op->mass_coeff == MG_COEFF_P0) ? c->field[MG_FIELD_MASS] : nullptr;
+  static const PermPack pp = pack_perm();
   for (const RowSegment &sg : S.segs) {
     const int nblk = (sg.row_end - sg.row_begin + 63) / 64;
     const size_t lds = sizeof(T) * (size_t)(sg.lds_slots + 1);
     const Rec *rec = (const Rec *)S.inc_rec;
 #define MG_LAUNCH(LR, MS)                                                                     \
   k_asm_rows<NL, LR, T, MS, Rec><<<nblk, 64, lds, c->stream>>>(                               \
-      sg.row_begin, sg.row_end, sg.lds_slots, S.rowptr, S.inc_ptr, rec, c->tets, c->xyz, ksc, \
+      sg.row_begin, sg.row_end, sg.lds_slots, pp, S.rowptr, S.inc_ptr, rec, c->tets, c->xyz,  \
+      ksc,                                                                                    \
       ksf, kmc, kmf, vals)
     if (sg.lrow == 0) {
       if (mass)
@@ -281,23 +359,11 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals) {
 
 static bool rule_ok(int r) { return r == 1 || r == 4 || r == 11 || r == 15; }
 
-static void upload_perm(Ctx *c) {
-  static bool done[64] = {false};
-  if (c->device < 64 && done[c->device]) return;
-  int perm[10][4], sigma[10][10];
-  relabel_tables(perm, sigma);
-  MG_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_perm), perm, sizeof(perm), 0, hipMemcpyHostToDevice,
-                                c->stream));
-  MG_HIP(hipStreamSynchronize(c->stream));
-  if (c->device < 64) done[c->device] = true;
-}
-
 template <typename T>
 static void assemble_t(Ctx *c, int sys, const mg_operator *op, int prec) {
   System &S = c->sys[sys];
   const bool stiff = op->terms & MG_TERM_STIFF, mass = op->terms & MG_TERM_MASS;
   upload_tables(c, c->order, stiff ? op->stiff_rule : 0, mass ? op->mass_rule : 0, prec);
-  upload_perm(c);
   T *vals = prec == MG_PREC_F64 ? (T *)S.vals64 : (T *)S.vals32;
   if (c->nloc == 10)
     launch_rows<10, T, RecP2>(c, S, op, vals);
