@@ -245,7 +245,7 @@ __global__ __launch_bounds__(64) void k_asm_rows(
   extern __shared__ __align__(16) char lds_raw[];
   T *acc = reinterpret_cast<T *>(lds_raw);
   const int lane = threadIdx.x;
-  const int r0 = row_begin + blockIdx.x * 64;
+  const int r0 = row_begin + xcd_chunk(blockIdx.x, gridDim.x) * 64;
   const int r1 = min(r0 + 64, row_end);
   const int s0 = rowptr[r0];
   const int ns = rowptr[r1] - s0;

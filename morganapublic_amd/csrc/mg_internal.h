@@ -135,6 +135,16 @@ void *scratch(Ctx *c, size_t bytes);  // context-owned reusable temporary
 
 inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X (each
+// with its own L2).  xcd_chunk maps the hardware block index to a logical
+// chunk index so that XCD k works on the k-th contiguous eighth of the chunks:
+// consecutive rows (and the vector entries / mesh data they touch) then stay
+// in one L2.  A bijection on [0, nblocks).
+__device__ __forceinline__ int xcd_chunk(int b, int nblocks) {
+  const int q = nblocks >> 3;
+  return b < (q << 3) ? (b & 7) * q + (b >> 3) : b;
+}
+
 // ---- cross-unit entry points ------------------------------------------------
 void build_edges(Ctx *c);
 void build_dofs(Ctx *c);
