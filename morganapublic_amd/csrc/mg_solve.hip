@@ -1,8 +1,7 @@
 // CSR SpMV and Jacobi-preconditioned CG (FP64 plain, or FP32 inner solves
 // inside an FP64 iterative refinement).
 //
-// One CG iteration is three kernels over a fixed grid (contiguous rows per
-// workgroup, XCD-contiguous):
+// One CG iteration is three kernels, all grid-stride over a fixed grid:
 //   k_spmv_pq : q = A p and the per-workgroup partials of p.q
 //   k_update  : alpha, x += alpha p, r -= alpha q, partials of r.D^{-1}r, r.r
 //   k_pupdate : convergence test, beta, p = D^{-1} r + beta p
@@ -32,20 +31,6 @@ struct PcgState {
 struct Partials {
   double *pq, *rz0, *rz1, *rr, *tmp;
 };
-
-// Contiguous rows of this workgroup: the grid splits [0, n) into gridDim.x
-// equal ranges, assigned XCD-contiguously (xcd_chunk).
-struct RowRange {
-  int lo, hi;
-};
-__device__ __forceinline__ RowRange block_rows(int n) {
-  const int per = (n + gridDim.x - 1) / gridDim.x;
-  const int c = xcd_chunk(blockIdx.x, gridDim.x);
-  RowRange r;
-  r.lo = min(n, c * per);
-  r.hi = min(n, r.lo + per);
-  return r;
-}
 
 // Every thread of the block gets the same total of one value per thread
 // (fixed shuffle tree + fixed order over the waves).
@@ -106,8 +91,8 @@ __global__ __launch_bounds__(kThreads) void k_spmv(int n, const int32_t *__restr
                                                    const T *__restrict__ vals,
                                                    const T *__restrict__ x, T *__restrict__ y) {
   const int lane = threadIdx.x % LPR;
-  const RowRange rr_ = block_rows(n);
-  for (int row = rr_.lo + threadIdx.x / LPR; row < rr_.hi; row += kThreads / LPR) {
+  const int stride = gridDim.x * (kThreads / LPR);
+  for (int row = blockIdx.x * (kThreads / LPR) + threadIdx.x / LPR; row < n; row += stride) {
     const T s = row_dot<T, LPR>(row, lane, rowptr, colidx, vals, x);
     if (lane == 0) y[row] = s;
   }
@@ -124,9 +109,9 @@ __global__ __launch_bounds__(kThreads) void k_spmv_pq(int n, const int32_t *__re
   __shared__ double sh[kThreads / 64];
   if (st->done) return;
   const int lane = threadIdx.x % LPR;
+  const int stride = gridDim.x * (kThreads / LPR);
   double acc = 0.0;
-  const RowRange rr_ = block_rows(n);
-  for (int row = rr_.lo + threadIdx.x / LPR; row < rr_.hi; row += kThreads / LPR) {
+  for (int row = blockIdx.x * (kThreads / LPR) + threadIdx.x / LPR; row < n; row += stride) {
     const T s = row_dot<T, LPR>(row, lane, rowptr, colidx, vals, p);
     if (lane == 0) {
       q[row] = s;
@@ -162,8 +147,7 @@ __global__ __launch_bounds__(kThreads) void k_update(int n, T *__restrict__ x, T
   }
   const T alpha = (T)(rz / pq);
   double a1 = 0.0, a2 = 0.0;
-  const RowRange rv_ = block_rows(n);
-  for (int i = rv_.lo + threadIdx.x; i < rv_.hi; i += kThreads) {
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
     x[i] += alpha * p[i];
     const T ri = r[i] - alpha * q[i];
     r[i] = ri;
@@ -220,8 +204,8 @@ __global__ __launch_bounds__(kThreads) void k_pupdate(int n, const T *__restrict
     return;
   }
   const T beta = (T)(rz_new / rz);
-  const RowRange rv_ = block_rows(n);
-  for (int i = rv_.lo + threadIdx.x; i < rv_.hi; i += kThreads) p[i] = dinv[i] * r[i] + beta * p[i];
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+    p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
 // r = b - A x (or r = b when x is null), p = D^{-1} r; partials r.p, r.r
@@ -236,9 +220,9 @@ __global__ __launch_bounds__(kThreads) void k_start(int n, const int32_t *__rest
                                                     double *__restrict__ rr_out) {
   __shared__ double sh[kThreads / 64];
   const int lane = threadIdx.x % LPR;
+  const int stride = gridDim.x * (kThreads / LPR);
   double a1 = 0.0, a2 = 0.0;
-  const RowRange rr_ = block_rows(n);
-  for (int row = rr_.lo + threadIdx.x / LPR; row < rr_.hi; row += kThreads / LPR) {
+  for (int row = blockIdx.x * (kThreads / LPR) + threadIdx.x / LPR; row < n; row += stride) {
     T ax = (T)0;
     if (x) ax = row_dot<T, LPR>(row, lane, rowptr, colidx, vals, x);
     if (lane == 0) {
@@ -285,9 +269,9 @@ __global__ __launch_bounds__(kThreads) void k_resid64(int n, const int32_t *__re
                                                       double *__restrict__ part) {
   __shared__ double sh[kThreads / 64];
   const int lane = threadIdx.x % LPR;
+  const int stride = gridDim.x * (kThreads / LPR);
   double rr = 0.0;
-  const RowRange rr_ = block_rows(n);
-  for (int row = rr_.lo + threadIdx.x / LPR; row < rr_.hi; row += kThreads / LPR) {
+  for (int row = blockIdx.x * (kThreads / LPR) + threadIdx.x / LPR; row < n; row += stride) {
     const double ax = row_dot<double, LPR>(row, lane, rowptr, colidx, vals, x);
     if (lane == 0) {
       const double ri = b[row] - ax;
@@ -302,8 +286,8 @@ __global__ __launch_bounds__(kThreads) void k_resid64(int n, const int32_t *__re
 __global__ void k_dot_self(int n, const double *__restrict__ v, double *__restrict__ part) {
   __shared__ double sh[kThreads / 64];
   double a = 0.0;
-  const RowRange rv_ = block_rows(n);
-  for (int i = rv_.lo + threadIdx.x; i < rv_.hi; i += kThreads) a += v[i] * v[i];
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+    a += v[i] * v[i];
   const double s = block_sum_all(a, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
