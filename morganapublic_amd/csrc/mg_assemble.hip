@@ -11,7 +11,9 @@
 // with J, det J (signed, as geoMapInterface::getGradientDet) and the adjugate
 // computed in FP64 from the vertex coordinates, and tab = the quadrature of
 // integratorOEL3d_STD contracted once on the reference element (constant
-// memory, compile-time indices -> scalar loads).  The contributions go into the
+// memory, compile-time indices -> scalar loads).  When the rule is exact on
+// the integrand, the same row comes from its closed form in barycentric
+// gradients (exact_row), about a third of the arithmetic.  The contributions go into the
 // row's entries staged in LDS; a row belongs to one lane and its incidences
 // are visited in a fixed order, so the sums are deterministic without atomics.
 // The workgroup then writes its contiguous CSR value range with coalesced
@@ -227,6 +229,100 @@ __device__ __forceinline__ void elem_factors(const Coords &c, double g[7]) {
   g[6] = c.km * det;
 }
 
+// ---- closed form of the stiffness row for quadrature rules exact on it ------
+// With n_j = det J grad(lambda_j) (n_1 = e2 x e3, n_2 = e3 x e1, n_3 = e1 x e2,
+// n_0 = -(n_1 + n_2 + n_3), e_k = P_k - P_0) and D_pq = n_p . n_q, the
+// stiffness entry is k/(6 det) * Q_lm(D), where Q integrates the products of
+// the basis gradients with  int lambda_p = vol/4  and
+// int lambda_p lambda_q = vol (1 + delta_pq) / 20:
+//   P1:                  Q_ij = D_ij
+//   P2 vertex i, vertex j: D_ij ((4/5)(1 + delta_ij) - 1)
+//   P2 vertex i, edge ab:  4 (D_ib E(i,a) + D_ia E(i,b)),  E(i,a) = (1 + delta_ia)/5 - 1/4
+//   P2 edge ab, edge cd:   16 (D_bd F(a,c) + D_bc F(a,d) + D_ad F(b,c) + D_ac F(b,d)),
+//                          F(p,q) = (1 + delta_pq)/20
+// This is what any quadrature rule exact on the integrand (P1: all rules;
+// P2: 4, 11, 15 points) returns, up to rounding.
+__host__ __device__ constexpr int edge_a(int j) {
+  return j == 0 ? 0 : j == 1 ? 1 : j == 2 ? 2 : j == 3 ? 0 : j == 4 ? 1 : 2;
+}
+__host__ __device__ constexpr int edge_b(int j) {
+  return j == 0 ? 1 : j == 1 ? 2 : j == 2 ? 0 : j == 3 ? 3 : j == 4 ? 3 : 3;
+}
+__host__ __device__ constexpr double dlt(int p, int q) { return p == q ? 1.0 : 0.0; }
+__host__ __device__ constexpr double coefE(int i, int a) { return (1.0 + dlt(i, a)) / 5.0 - 0.25; }
+__host__ __device__ constexpr double coefF(int p, int q) { return (1.0 + dlt(p, q)) / 20.0; }
+
+// D[p][q] for p in {0, 1}, q in 0..3 of the relabelled element, and det
+__device__ __forceinline__ void bary_dots(const Coords &c, bool need_row1, double D0[4],
+                                          double D1[4], double &det) {
+  double e[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) e[k][r] = c.x[3 * (k + 1) + r] - c.x[r];
+  double n[4][3];
+  n[1][0] = e[1][1] * e[2][2] - e[1][2] * e[2][1];
+  n[1][1] = e[1][2] * e[2][0] - e[1][0] * e[2][2];
+  n[1][2] = e[1][0] * e[2][1] - e[1][1] * e[2][0];
+  n[2][0] = e[2][1] * e[0][2] - e[2][2] * e[0][1];
+  n[2][1] = e[2][2] * e[0][0] - e[2][0] * e[0][2];
+  n[2][2] = e[2][0] * e[0][1] - e[2][1] * e[0][0];
+  n[3][0] = e[0][1] * e[1][2] - e[0][2] * e[1][1];
+  n[3][1] = e[0][2] * e[1][0] - e[0][0] * e[1][2];
+  n[3][2] = e[0][0] * e[1][1] - e[0][1] * e[1][0];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) n[0][r] = -(n[1][r] + n[2][r] + n[3][r]);
+  det = e[0][0] * n[1][0] + e[0][1] * n[1][1] + e[0][2] * n[1][2];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) D0[q] = n[0][0] * n[q][0] + n[0][1] * n[q][1] + n[0][2] * n[q][2];
+  if (need_row1) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) D1[q] = n[1][0] * n[q][0] + n[1][1] * n[q][1] + n[1][2] * n[q][2];
+  }
+}
+
+// Row LROW (0: vertex, 4: edge (0,1)) of the relabelled element matrix, exact.
+template <int NL, int LROW, bool MASS, typename T>
+__device__ __forceinline__ void exact_row(const Coords &c, T v[NL]) {
+  double D0[4], D1[4], det;
+  bary_dots(c, LROW == 4, D0, D1, det);
+  const double s = c.ks / (6.0 * det);
+  const double md = c.km * det;
+  if (NL == 4) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double val = s * D0[j];
+      if (MASS) val += md * tab<double>((6 * NL + LROW) * NL + j);
+      v[j] = (T)val;
+    }
+    return;
+  }
+#pragma unroll
+  for (int m = 0; m < NL; ++m) {
+    double q;
+    if (LROW == 0) {
+      if (m < 4) {
+        q = D0[m] * (0.8 * (1.0 + dlt(0, m)) - 1.0);
+      } else {
+        const int a = edge_a(m - 4), b = edge_b(m - 4);
+        q = 4.0 * (D0[b] * coefE(0, a) + D0[a] * coefE(0, b));
+      }
+    } else {
+      if (m < 4) {
+        q = 4.0 * (D1[m] * coefE(m, 0) + D0[m] * coefE(m, 1));
+      } else {
+        const int cc = edge_a(m - 4), d = edge_b(m - 4);
+        // row edge (a,b) = (0,1): D_bd F(a,c) + D_bc F(a,d) + D_ad F(b,c) + D_ac F(b,d)
+        q = 16.0 * (D1[d] * coefF(0, cc) + D1[cc] * coefF(0, d) + D0[d] * coefF(1, cc) +
+                    D0[cc] * coefF(1, d));
+      }
+    }
+    double val = s * q;
+    if (MASS) val += md * tab<double>((6 * NL + LROW) * NL + m);
+    v[m] = (T)val;
+  }
+}
+
 // One lane per CSR row, 64 rows per workgroup, the block's entries staged in
 // LDS.  A lane walks its row's incidences in their fixed order; each element
 // is relabelled so the row is its local basis LROW, and row LROW of the element
@@ -234,7 +330,7 @@ __device__ __forceinline__ void elem_factors(const Coords &c, double g[7]) {
 // loads).  Software pipeline over the incidences: while incidence j is
 // contracted, the coordinates of j+1, the vertex ids of j+2 and the record of
 // j+3 are in flight.
-template <int NL, int LROW, typename T, bool MASS, typename Rec>
+template <int NL, int LROW, typename T, bool MASS, bool EXACT, typename Rec>
 __global__ __launch_bounds__(64) void k_asm_rows(
     int32_t row_begin, int32_t row_end, int32_t dummy, PermPack pp,
     const int32_t *__restrict__ rowptr,
@@ -280,17 +376,23 @@ __global__ __launch_bounds__(64) void k_asm_rows(
       }
       C.t = load_tet(tets, C.r.el);
       const bool valid = j < iend;
-      double g[7];
-      elem_factors(A.c, g);
       T v[NL];
+      if (EXACT) {
+        exact_row<NL, LROW, MASS, T>(A.c, v);
+      } else {
+        double g[7];
+        elem_factors(A.c, g);
 #pragma unroll
-      for (int m = 0; m < NL; ++m) {
-        T sum = (T)g[0] * tab<T>((0 * NL + LROW) * NL + m);
+        for (int m = 0; m < NL; ++m) {
+          T sum = (T)g[0] * tab<T>((0 * NL + LROW) * NL + m);
 #pragma unroll
-        for (int k = 1; k < 6; ++k) sum += (T)g[k] * tab<T>((k * NL + LROW) * NL + m);
-        if (MASS) sum += (T)g[6] * tab<T>((6 * NL + LROW) * NL + m);
-        v[m] = valid ? sum : (T)0;
+          for (int k = 1; k < 6; ++k) sum += (T)g[k] * tab<T>((k * NL + LROW) * NL + m);
+          if (MASS) sum += (T)g[6] * tab<T>((6 * NL + LROW) * NL + m);
+          v[m] = sum;
+        }
       }
+#pragma unroll
+      for (int m = 0; m < NL; ++m) v[m] = valid ? v[m] : (T)0;
       int idx[NL];
 #pragma unroll
       for (int m = 0; m < NL; ++m) {
@@ -323,35 +425,48 @@ __global__ __launch_bounds__(64) void k_asm_rows(
   for (int s = lane; s < ns; s += 64) vals[s0 + s] = acc[s];
 }
 
+// rules exact on the term's integrand (affine element, P0 coefficient)
+static bool rule_exact(int order, int term, int rule) {
+  if (term == MG_TERM_STIFF) return order == 1 ? rule >= 1 : rule >= 4;
+  return order == 1 ? rule >= 4 : rule >= 11;  // mass: degree 2 / degree 4
+}
+
 template <int NL, typename T, typename Rec>
 static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals) {
   const bool stiff = op->terms & MG_TERM_STIFF, mass = op->terms & MG_TERM_MASS;
   const double ksc = stiff ? op->stiff_const : 0.0, kmc = mass ? op->mass_const : 0.0;
   const double *ksf = (stiff && op->stiff_coeff == MG_COEFF_P0) ? c->field[MG_FIELD_STIFF] : nullptr;
   const double *kmf = (mass && op->mass_coeff == MG_COEFF_P0) ? c->field[MG_FIELD_MASS] : nullptr;
+  const bool exact = (!stiff || rule_exact(c->order, MG_TERM_STIFF, op->stiff_rule)) &&
+                     (!mass || rule_exact(c->order, MG_TERM_MASS, op->mass_rule));
   static const PermPack pp = pack_perm();
   for (const RowSegment &sg : S.segs) {
     const int nblk = (sg.row_end - sg.row_begin + 63) / 64;
     const size_t lds = sizeof(T) * (size_t)(sg.lds_slots + 1);
     const Rec *rec = (const Rec *)S.inc_rec;
-#define MG_LAUNCH(LR, MS)                                                                     \
-  k_asm_rows<NL, LR, T, MS, Rec><<<nblk, 64, lds, c->stream>>>(                               \
-      sg.row_begin, sg.row_end, sg.lds_slots, pp, S.rowptr, S.inc_ptr, rec, c->tets, c->xyz,  \
-      ksc,                                                                                    \
-      ksf, kmc, kmf, vals)
-    if (sg.lrow == 0) {
-      if (mass)
-        MG_LAUNCH(0, true);
-      else
-        MG_LAUNCH(0, false);
-    } else {
-      if (NL == 10) {
-        if (mass)
-          MG_LAUNCH(4, true);
-        else
-          MG_LAUNCH(4, false);
-      }
-    }
+#define MG_LAUNCH(LR, MS, EX)                                                                  \
+  k_asm_rows<NL, LR, T, MS, EX, Rec><<<nblk, 64, lds, c->stream>>>(                            \
+      sg.row_begin, sg.row_end, sg.lds_slots, pp, S.rowptr, S.inc_ptr, rec, c->tets, c->xyz,   \
+      ksc, ksf, kmc, kmf, vals)
+#define MG_LAUNCH2(LR)          \
+  do {                          \
+    if (exact) {                \
+      if (mass)                 \
+        MG_LAUNCH(LR, true, true); \
+      else                      \
+        MG_LAUNCH(LR, false, true); \
+    } else {                    \
+      if (mass)                 \
+        MG_LAUNCH(LR, true, false); \
+      else                      \
+        MG_LAUNCH(LR, false, false); \
+    }                           \
+  } while (0)
+    if (sg.lrow == 0)
+      MG_LAUNCH2(0);
+    else if (NL == 10)
+      MG_LAUNCH2(4);
+#undef MG_LAUNCH2
 #undef MG_LAUNCH
     MG_HIP(hipGetLastError());
   }

@@ -34,6 +34,9 @@ struct RowSegment {
   int32_t row_begin, row_end, lds_slots, lrow;
 };
 
+constexpr int kSpmvRows = 64;
+constexpr int kSpmvNnz = 2048;
+
 struct System {
   bool built = false;
   int64_t nrows = 0, nnz = 0, n_inc = 0;
@@ -45,6 +48,10 @@ struct System {
   int32_t *inc_ptr = nullptr;  // nrows + 1 (row -> incidences)
   void *inc_rec = nullptr;     // n_inc records (RecP1 or RecP2), sorted by (row, l, e)
   std::vector<RowSegment> segs;
+  // SpMV row blocks: rows [sblk[b], sblk[b+1]) hold <= kSpmvNnz entries and
+  // <= kSpmvRows rows
+  int32_t *sblk = nullptr;
+  int32_t n_sblk = 0;
   double *vals64 = nullptr;
   float *vals32 = nullptr;
   bool has64 = false, has32 = false;

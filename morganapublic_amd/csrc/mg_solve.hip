@@ -85,6 +85,39 @@ __device__ __forceinline__ T row_dot(int row, int lane, const int32_t *__restric
   return s;
 }
 
+// CSR-stream product of one row block: the block's entries are streamed by
+// all 256 threads (coalesced) into LDS as vals * x[col], then 4 lanes per row
+// sum their row in a fixed order.  Returns the row value in lane sub == 0 of
+// each 4-lane group (row index in *row_out, -1 when the group has no row).
+template <typename T>
+__device__ __forceinline__ T stream_block(int b, const int32_t *__restrict__ sblk,
+                                          const int32_t *__restrict__ rowptr,
+                                          const int32_t *__restrict__ colidx,
+                                          const T *__restrict__ vals, const T *__restrict__ x,
+                                          T *prod, int *row_out) {
+  const int r0 = sblk[b], r1 = sblk[b + 1];
+  const int s0 = rowptr[r0], ns = rowptr[r1] - s0;
+  constexpr int PER = kSpmvNnz / kThreads;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int k = threadIdx.x + u * kThreads;
+    if (k < ns) prod[k] = vals[s0 + k] * x[colidx[s0 + k]];
+  }
+  __syncthreads();
+  const int g = threadIdx.x >> 2, sub = threadIdx.x & 3;
+  const int row = r0 + g;
+  T sum = (T)0;
+  if (row < r1) {
+    const int a = rowptr[row] - s0, e = rowptr[row + 1] - s0;
+    for (int k = a + sub; k < e; k += 4) sum += prod[k];
+  }
+  sum += __shfl_xor(sum, 1, 4);
+  sum += __shfl_xor(sum, 2, 4);
+  __syncthreads();
+  *row_out = (row < r1) ? row : -1;
+  return sum;
+}
+
 template <typename T, int LPR>
 __global__ __launch_bounds__(kThreads) void k_spmv(int n, const int32_t *__restrict__ rowptr,
                                                    const int32_t *__restrict__ colidx,
@@ -120,6 +153,66 @@ __global__ __launch_bounds__(kThreads) void k_spmv_pq(int n, const int32_t *__re
   }
   const double b = block_sum_all(acc, sh);
   if (threadIdx.x == 0) pq_part[blockIdx.x] = b;
+}
+
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_spmv_stream(int nb, const int32_t *__restrict__ sblk,
+                                                          const int32_t *__restrict__ rowptr,
+                                                          const int32_t *__restrict__ colidx,
+                                                          const T *__restrict__ vals,
+                                                          const T *__restrict__ x,
+                                                          T *__restrict__ y) {
+  __shared__ T prod[kSpmvNnz];
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    int row;
+    const T s = stream_block<T>(b, sblk, rowptr, colidx, vals, x, prod, &row);
+    if ((threadIdx.x & 3) == 0 && row >= 0) y[row] = s;
+  }
+}
+
+// q = A p; partial p.q (row blocks)
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_spmv_pq_stream(
+    int nb, const int32_t *__restrict__ sblk, const int32_t *__restrict__ rowptr,
+    const int32_t *__restrict__ colidx, const T *__restrict__ vals, const T *__restrict__ p,
+    T *__restrict__ q, double *__restrict__ pq_part, const PcgState *__restrict__ st) {
+  __shared__ T prod[kSpmvNnz];
+  __shared__ double sh[kThreads / 64];
+  if (st->done) return;
+  double acc = 0.0;
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    int row;
+    const T s = stream_block<T>(b, sblk, rowptr, colidx, vals, p, prod, &row);
+    if ((threadIdx.x & 3) == 0 && row >= 0) {
+      q[row] = s;
+      acc += (double)(s * p[row]);
+    }
+  }
+  const double bsum = block_sum_all(acc, sh);
+  if (threadIdx.x == 0) pq_part[blockIdx.x] = bsum;
+}
+
+// FP64 residual r = b - A x; partial r.r (row blocks)
+__global__ __launch_bounds__(kThreads) void k_resid64_stream(
+    int nb, const int32_t *__restrict__ sblk, const int32_t *__restrict__ rowptr,
+    const int32_t *__restrict__ colidx, const double *__restrict__ vals,
+    const double *__restrict__ x, const double *__restrict__ b, double *__restrict__ r,
+    double *__restrict__ part) {
+  __shared__ double prod[kSpmvNnz];
+  __shared__ double sh[kThreads / 64];
+  double rr = 0.0;
+  for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {
+    int row;
+    const double ax = stream_block<double>(blk, sblk, rowptr, colidx, vals, x, prod, &row);
+    if ((threadIdx.x & 3) == 0 && row >= 0) {
+      const double ri = b[row] - ax;
+      r[row] = ri;
+      rr += ri * ri;
+    }
+  }
+  const double s = block_sum_all(rr, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 // alpha = rz / pq; x += alpha p; r -= alpha q; partials r.D^{-1}r and r.r
@@ -360,13 +453,14 @@ void ensure_dinv(Ctx *c, int sys, int prec) {
   MG_HIP(hipGetLastError());
 }
 
+static int grid_blocks(const System &S) {
+  return S.n_sblk < kMaxBlocks ? (S.n_sblk < 1 ? 1 : S.n_sblk) : kMaxBlocks;
+}
+
 template <typename T>
 static void spmv_t(Ctx *c, System &S, const T *vals, const T *x, T *y) {
-  const int n = (int)S.nrows;
-  if (lanes_per_row(S) == 8)
-    k_spmv<T, 8><<<grid_rows(n, 8), kThreads, 0, c->stream>>>(n, S.rowptr, S.colidx, vals, x, y);
-  else
-    k_spmv<T, 4><<<grid_rows(n, 4), kThreads, 0, c->stream>>>(n, S.rowptr, S.colidx, vals, x, y);
+  k_spmv_stream<T><<<grid_blocks(S), kThreads, 0, c->stream>>>(S.n_sblk, S.sblk, S.rowptr,
+                                                               S.colidx, vals, x, y);
   MG_HIP(hipGetLastError());
 }
 
@@ -455,7 +549,7 @@ static PcgState pcg_core(Ctx *c, System &S, const T *vals, const T *dinv, const 
                          Workspace &w) {
   const int n = (int)S.nrows;
   init_state(c, w.st, tol2, maxit);
-  const int gr = grid_rows(n, LPR), gv = grid_vec(n);
+  const int gr = grid_rows(n, LPR), gv = grid_vec(n), gb = grid_blocks(S);
   if (x_is_zero) MG_HIP(hipMemsetAsync(x, 0, sizeof(T) * n, c->stream));
   Partials &P = w.part;
   // k_start writes the r.z partials where iteration 0 reads its "previous" r.z
@@ -478,9 +572,9 @@ static PcgState pcg_core(Ctx *c, System &S, const T *vals, const T *dinv, const 
     for (int k = 0; k < chunk; ++k, ++it) {
       double *rz_prev = (it & 1) ? P.rz0 : P.rz1;
       double *rz_new = (it & 1) ? P.rz1 : P.rz0;
-      k_spmv_pq<T, LPR><<<gr, kThreads, 0, c->stream>>>(n, S.rowptr, S.colidx, vals, p, q, P.pq,
-                                                        w.st);
-      k_update<T><<<gv, kThreads, 0, c->stream>>>(n, x, r, p, q, dinv, P.pq, gr, rz_prev, gv,
+      k_spmv_pq_stream<T><<<gb, kThreads, 0, c->stream>>>(S.n_sblk, S.sblk, S.rowptr, S.colidx,
+                                                         vals, p, q, P.pq, w.st);
+      k_update<T><<<gv, kThreads, 0, c->stream>>>(n, x, r, p, q, dinv, P.pq, gb, rz_prev, gv,
                                                   rz_new, P.rr, w.st);
       k_pupdate<T><<<gv, kThreads, 0, c->stream>>>(n, r, p, dinv, rz_new, rz_prev, P.rr, gv, it,
                                                    w.st);
@@ -503,16 +597,10 @@ static PcgState pcg_dispatch(Ctx *c, System &S, const T *vals, const T *dinv, co
 static double true_relres(Ctx *c, System &S, const double *b, const double *x, Workspace &w,
                           double bnorm) {
   const int n = (int)S.nrows;
-  int g;
-  if (lanes_per_row(S) == 8) {
-    g = grid_rows(n, 8);
-    k_resid64<8><<<g, kThreads, 0, c->stream>>>(n, S.rowptr, S.colidx, S.vals64, x, b, w.rr64,
-                                                w.part.tmp);
-  } else {
-    g = grid_rows(n, 4);
-    k_resid64<4><<<g, kThreads, 0, c->stream>>>(n, S.rowptr, S.colidx, S.vals64, x, b, w.rr64,
-                                                w.part.tmp);
-  }
+  (void)n;
+  const int g = grid_blocks(S);
+  k_resid64_stream<<<g, kThreads, 0, c->stream>>>(S.n_sblk, S.sblk, S.rowptr, S.colidx, S.vals64,
+                                                  x, b, w.rr64, w.part.tmp);
   k_finalize_sum<<<1, kThreads, 0, c->stream>>>(w.part.tmp, g, &w.st->rr);
   MG_HIP(hipGetLastError());
   PcgState h = read_state(c, w.st);

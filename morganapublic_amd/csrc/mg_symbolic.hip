@@ -438,6 +438,24 @@ void build_pattern(Ctx *c, int sys) {
   const int64_t nvrows =
       nl == 4 ? nrows : (sys == MG_SYS_FULL ? c->nV : c->nfree_vertex);
   make_segments(S, rp, nvrows);
+  {
+    // SpMV row blocks
+    std::vector<int32_t> sb;
+    int64_t r = 0;
+    sb.push_back(0);
+    while (r < nrows) {
+      int64_t r1 = r + 1;
+      MG_CHECK(rp[r + 1] - rp[r] <= kSpmvNnz, MG_ERR_SIZE, "pattern: row longer than 2048");
+      while (r1 < nrows && r1 - r < kSpmvRows && rp[r1 + 1] - rp[r] <= kSpmvNnz) ++r1;
+      sb.push_back((int32_t)r1);
+      r = r1;
+    }
+    S.n_sblk = (int32_t)sb.size() - 1;
+    S.sblk = dalloc<int32_t>(c, sb.size());
+    MG_HIP(hipMemcpyAsync(S.sblk, sb.data(), sizeof(int32_t) * sb.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    MG_HIP(hipStreamSynchronize(c->stream));
+  }
   for (const RowSegment &sg : S.segs)
     MG_CHECK(sg.lds_slots * 8 <= 64 * 1024 - 64, MG_ERR_SIZE,
              "pattern: a 64-row block owns more than 8184 CSR entries");
