@@ -330,8 +330,8 @@ __device__ __forceinline__ void exact_row(const Coords &c, T v[NL]) {
 // loads).  Software pipeline over the incidences: while incidence j is
 // contracted, the coordinates of j+1, the vertex ids of j+2 and the record of
 // j+3 are in flight.
-template <int NL, int LROW, typename T, bool MASS, bool EXACT, typename Rec>
-__global__ __launch_bounds__(64) void k_asm_rows(
+template <int NL, int LROW, typename T, bool MASS, bool EXACT, int ROWS, typename Rec>
+__global__ __launch_bounds__(ROWS) void k_asm_rows(
     int32_t row_begin, int32_t row_end, int32_t dummy, PermPack pp,
     const int32_t *__restrict__ rowptr,
     const int32_t *__restrict__ inc_ptr, const Rec *__restrict__ rec,
@@ -341,11 +341,11 @@ __global__ __launch_bounds__(64) void k_asm_rows(
   extern __shared__ __align__(16) char lds_raw[];
   T *acc = reinterpret_cast<T *>(lds_raw);
   const int lane = threadIdx.x;
-  const int r0 = row_begin + xcd_chunk(blockIdx.x, gridDim.x) * 64;
-  const int r1 = min(r0 + 64, row_end);
+  const int r0 = row_begin + xcd_chunk(blockIdx.x, gridDim.x) * ROWS;
+  const int r1 = min(r0 + ROWS, row_end);
   const int s0 = rowptr[r0];
   const int ns = rowptr[r1] - s0;
-  for (int s = lane; s < ns; s += 64) acc[s] = (T)0;
+  for (int s = lane; s < ns; s += ROWS) acc[s] = (T)0;
   __syncthreads();
   const int r = r0 + lane;
   if (r < r1) {
@@ -422,7 +422,7 @@ __global__ __launch_bounds__(64) void k_asm_rows(
     }
   }
   __syncthreads();
-  for (int s = lane; s < ns; s += 64) vals[s0 + s] = acc[s];
+  for (int s = lane; s < ns; s += ROWS) vals[s0 + s] = acc[s];
 }
 
 // rules exact on the term's integrand (affine element, P0 coefficient)
@@ -441,13 +441,22 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals) {
                      (!mass || rule_exact(c->order, MG_TERM_MASS, op->mass_rule));
   static const PermPack pp = pack_perm();
   for (const RowSegment &sg : S.segs) {
-    const int nblk = (sg.row_end - sg.row_begin + 63) / 64;
+    const int nblk = (sg.row_end - sg.row_begin + sg.rows - 1) / sg.rows;
     const size_t lds = sizeof(T) * (size_t)(sg.lds_slots + 1);
     const Rec *rec = (const Rec *)S.inc_rec;
-#define MG_LAUNCH(LR, MS, EX)                                                                  \
-  k_asm_rows<NL, LR, T, MS, EX, Rec><<<nblk, 64, lds, c->stream>>>(                            \
+#define MG_LAUNCH1(LR, MS, EX, RW)                                                             \
+  k_asm_rows<NL, LR, T, MS, EX, RW, Rec><<<nblk, RW, lds, c->stream>>>(                        \
       sg.row_begin, sg.row_end, sg.lds_slots, pp, S.rowptr, S.inc_ptr, rec, c->tets, c->xyz,   \
       ksc, ksf, kmc, kmf, vals)
+#define MG_LAUNCH(LR, MS, EX)                \
+  do {                                       \
+    if (sg.rows == 256)                      \
+      MG_LAUNCH1(LR, MS, EX, 256);           \
+    else if (sg.rows == 128)                 \
+      MG_LAUNCH1(LR, MS, EX, 128);           \
+    else                                     \
+      MG_LAUNCH1(LR, MS, EX, 64);            \
+  } while (0)
 #define MG_LAUNCH2(LR)          \
   do {                          \
     if (exact) {                \
@@ -468,6 +477,7 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals) {
       MG_LAUNCH2(4);
 #undef MG_LAUNCH2
 #undef MG_LAUNCH
+#undef MG_LAUNCH1
     MG_HIP(hipGetLastError());
   }
 }

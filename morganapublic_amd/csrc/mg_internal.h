@@ -27,11 +27,11 @@ struct alignas(8) RecP1 {
   uint8_t pos[4];
 };
 
-// Rows of one assembly launch: 64-row blocks, each staging its CSR entries in
-// LDS; lrow is the local basis every incidence of these rows is relabelled to
+// Rows of one assembly launch: blocks of `rows` rows, each staging its CSR
+// entries in LDS; lrow is the local basis every incidence of these rows is relabelled to
 // (0 for vertex-dof rows, 4 for P2 edge-dof rows).
 struct RowSegment {
-  int32_t row_begin, row_end, lds_slots, lrow;
+  int32_t row_begin, row_end, lds_slots, lrow, rows;  // rows: rows (= lanes) per workgroup
 };
 
 constexpr int kSpmvRows = 64;

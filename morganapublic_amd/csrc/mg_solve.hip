@@ -98,10 +98,24 @@ __device__ __forceinline__ T stream_block(int b, const int32_t *__restrict__ sbl
   const int r0 = sblk[b], r1 = sblk[b + 1];
   const int s0 = rowptr[r0], ns = rowptr[r1] - s0;
   constexpr int PER = kSpmvNnz / kThreads;
+  if (ns > 0) {
+    // branch-free: every lane issues its PER (column, value) loads, then its PER
+    // gathers; lanes past the block's end re-read entry s0 and store into the
+    // spare LDS slot kSpmvNnz
+    int col[PER];
+    T v[PER];
 #pragma unroll
-  for (int u = 0; u < PER; ++u) {
-    const int k = threadIdx.x + u * kThreads;
-    if (k < ns) prod[k] = vals[s0 + k] * x[colidx[s0 + k]];
+    for (int u = 0; u < PER; ++u) {
+      const int k = threadIdx.x + u * kThreads;
+      const int kc = s0 + (k < ns ? k : 0);
+      col[u] = colidx[kc];
+      v[u] = vals[kc];
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int k = threadIdx.x + u * kThreads;
+      prod[k < ns ? k : kSpmvNnz] = v[u] * x[col[u]];
+    }
   }
   __syncthreads();
   const int g = threadIdx.x >> 2, sub = threadIdx.x & 3;
@@ -163,7 +177,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv_stream(int nb, const int32_t 
                                                           const T *__restrict__ vals,
                                                           const T *__restrict__ x,
                                                           T *__restrict__ y) {
-  __shared__ T prod[kSpmvNnz];
+  __shared__ T prod[kSpmvNnz + 1];
   for (int b = blockIdx.x; b < nb; b += gridDim.x) {
     int row;
     const T s = stream_block<T>(b, sblk, rowptr, colidx, vals, x, prod, &row);
@@ -177,7 +191,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv_pq_stream(
     int nb, const int32_t *__restrict__ sblk, const int32_t *__restrict__ rowptr,
     const int32_t *__restrict__ colidx, const T *__restrict__ vals, const T *__restrict__ p,
     T *__restrict__ q, double *__restrict__ pq_part, const PcgState *__restrict__ st) {
-  __shared__ T prod[kSpmvNnz];
+  __shared__ T prod[kSpmvNnz + 1];
   __shared__ double sh[kThreads / 64];
   if (st->done) return;
   double acc = 0.0;
@@ -199,7 +213,7 @@ __global__ __launch_bounds__(kThreads) void k_resid64_stream(
     const int32_t *__restrict__ colidx, const double *__restrict__ vals,
     const double *__restrict__ x, const double *__restrict__ b, double *__restrict__ r,
     double *__restrict__ part) {
-  __shared__ double prod[kSpmvNnz];
+  __shared__ double prod[kSpmvNnz + 1];
   __shared__ double sh[kThreads / 64];
   double rr = 0.0;
   for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {

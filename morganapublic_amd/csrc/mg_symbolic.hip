@@ -338,14 +338,23 @@ __global__ void k_inc_records(int64_t n_inc, const uint64_t *__restrict__ keys,
 // uses 64-row blocks and the LDS size of its largest block.
 static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvrows) {
   S.segs.clear();
-  auto add = [&](int64_t a, int64_t b, int lrow) {
+  const char *env = getenv("MGFE_EDGE_ROWS");
+  const int edge_rows = env ? atoi(env) : 64;
+  auto add = [&](int64_t a, int64_t b, int lrow, int rows) {
     if (b <= a) return;
-    int mx = 0;
-    for (int64_t r = a; r < b; r += 64) mx = std::max(mx, rp[std::min<int64_t>(r + 64, b)] - rp[r]);
-    S.segs.push_back(RowSegment{(int32_t)a, (int32_t)b, mx, lrow});
+    for (;;) {
+      int mx = 0;
+      for (int64_t r = a; r < b; r += rows)
+        mx = std::max(mx, rp[std::min<int64_t>(r + rows, b)] - rp[r]);
+      if ((mx + 1) * 8 <= 64 * 1024 || rows == 64) {
+        S.segs.push_back(RowSegment{(int32_t)a, (int32_t)b, mx, lrow, rows});
+        return;
+      }
+      rows /= 2;  // a block's entries must fit 64 KiB of LDS
+    }
   };
-  add(0, nvrows, 0);
-  add(nvrows, S.nrows, 4);
+  add(0, nvrows, 0, 64);
+  add(nvrows, S.nrows, 4, (edge_rows == 128 || edge_rows == 256) ? edge_rows : 64);
 }
 
 void build_pattern(Ctx *c, int sys) {
