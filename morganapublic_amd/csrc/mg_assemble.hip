@@ -330,8 +330,8 @@ __device__ __forceinline__ void exact_row(const Coords &c, T v[NL]) {
 // loads).  Software pipeline over the incidences: while incidence j is
 // contracted, the coordinates of j+1, the vertex ids of j+2 and the record of
 // j+3 are in flight.
-template <int NL, int LROW, typename T, bool MASS, bool EXACT, int ROWS, typename Rec>
-__global__ __launch_bounds__(ROWS) void k_asm_rows(
+template <int NL, int LROW, typename T, bool MASS, bool EXACT, int ROWS, int LPR, typename Rec>
+__global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
     int32_t row_begin, int32_t row_end, int32_t dummy, PermPack pp,
     const int32_t *__restrict__ rowptr,
     const int32_t *__restrict__ inc_ptr, const Rec *__restrict__ rec,
@@ -340,18 +340,20 @@ __global__ __launch_bounds__(ROWS) void k_asm_rows(
     T *__restrict__ vals) {
   extern __shared__ __align__(16) char lds_raw[];
   T *acc = reinterpret_cast<T *>(lds_raw);
+  constexpr int NT = ROWS * LPR;
   const int lane = threadIdx.x;
+  const int sub = lane % LPR;  // LPR lanes share a row and take every LPR-th incidence
   const int r0 = row_begin + xcd_chunk(blockIdx.x, gridDim.x) * ROWS;
   const int r1 = min(r0 + ROWS, row_end);
   const int s0 = rowptr[r0];
   const int ns = rowptr[r1] - s0;
-  for (int s = lane; s < ns; s += ROWS) acc[s] = (T)0;
+  for (int s = lane; s < ns; s += NT) acc[s] = (T)0;
   __syncthreads();
-  const int r = r0 + lane;
+  const int r = r0 + lane / LPR;
   if (r < r1) {
     const int base = rowptr[r] - s0;
-    int i = inc_ptr[r];
     const int iend = inc_ptr[r + 1];
+    int i = inc_ptr[r] + sub;
     // Three slots rotate through the stages (record -> vertex ids ->
     // coordinates -> contraction); each step issues the next stage of the two
     // following incidences and the record three ahead, then contracts the
@@ -359,8 +361,7 @@ __global__ __launch_bounds__(ROWS) void k_asm_rows(
     // consumes it and no loaded register is copied; the body has no branch:
     // indices past the row's last incidence are clamped (loads stay in
     // bounds) and their contribution goes to the dummy slot with value 0.
-    const int i0 = i;
-    const int last = iend > i0 ? iend - 1 : 0;  // a row without incidences reads record 0
+    const int last = iend > i - sub ? iend - 1 : 0;  // a row without incidences reads record 0
     auto at = [&](int j) { return j < iend ? j : last; };
     struct Slot {
       RecWords<NL> r;
@@ -399,12 +400,19 @@ __global__ __launch_bounds__(ROWS) void k_asm_rows(
         const int pm = A.r.pos(m);
         idx[m] = (pm == 0xFF || !valid) ? dummy : base + pm;
       }
-      T a[NL];
+      // the LPR lanes of a row add in turn (lane sub in phase sub), so every row
+      // receives its incidences in their stored order
 #pragma unroll
-      for (int m = 0; m < NL; ++m) a[m] = acc[idx[m]];
+      for (int ph = 0; ph < LPR; ++ph) {
+        if (LPR == 1 || sub == ph) {
+          T a[NL];
 #pragma unroll
-      for (int m = 0; m < NL; ++m) acc[idx[m]] = a[m] + v[m];
-      A.r = load_rec<NL>(rec, at(j + 3));
+          for (int m = 0; m < NL; ++m) a[m] = acc[idx[m]];
+#pragma unroll
+          for (int m = 0; m < NL; ++m) acc[idx[m]] = a[m] + v[m];
+        }
+      }
+      A.r = load_rec<NL>(rec, at(j + 3 * LPR));
     };
     S0.r = load_rec<NL>(rec, at(i));
     {
@@ -412,17 +420,17 @@ __global__ __launch_bounds__(ROWS) void k_asm_rows(
       relabel(load_tet(tets, S0.r.el), S0.r.el, pp, id);
       S0.c = load_coords(xyz, id, S0.r.el >> 4, ks_const, ks_field, km_const, km_field);
     }
-    S1.r = load_rec<NL>(rec, at(i + 1));
+    S1.r = load_rec<NL>(rec, at(i + LPR));
     S1.t = load_tet(tets, S1.r.el);
-    S2.r = load_rec<NL>(rec, at(i + 2));
-    for (; i < iend; i += 3) {
+    S2.r = load_rec<NL>(rec, at(i + 2 * LPR));
+    for (; i < iend; i += 3 * LPR) {
       step(S0, S1, S2, i);
-      step(S1, S2, S0, i + 1);
-      step(S2, S0, S1, i + 2);
+      step(S1, S2, S0, i + LPR);
+      step(S2, S0, S1, i + 2 * LPR);
     }
   }
   __syncthreads();
-  for (int s = lane; s < ns; s += ROWS) vals[s0 + s] = acc[s];
+  for (int s = lane; s < ns; s += NT) vals[s0 + s] = acc[s];
 }
 
 // rules exact on the term's integrand (affine element, P0 coefficient)
@@ -444,18 +452,26 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals) {
     const int nblk = (sg.row_end - sg.row_begin + sg.rows - 1) / sg.rows;
     const size_t lds = sizeof(T) * (size_t)(sg.lds_slots + 1);
     const Rec *rec = (const Rec *)S.inc_rec;
-#define MG_LAUNCH1(LR, MS, EX, RW)                                                             \
-  k_asm_rows<NL, LR, T, MS, EX, RW, Rec><<<nblk, RW, lds, c->stream>>>(                        \
+#define MG_LAUNCH1(LR, MS, EX, RW, LP)                                                         \
+  k_asm_rows<NL, LR, T, MS, EX, RW, LP, Rec><<<nblk, RW * LP, lds, c->stream>>>(               \
       sg.row_begin, sg.row_end, sg.lds_slots, pp, S.rowptr, S.inc_ptr, rec, c->tets, c->xyz,   \
       ksc, ksf, kmc, kmf, vals)
-#define MG_LAUNCH(LR, MS, EX)                \
-  do {                                       \
-    if (sg.rows == 256)                      \
-      MG_LAUNCH1(LR, MS, EX, 256);           \
-    else if (sg.rows == 128)                 \
-      MG_LAUNCH1(LR, MS, EX, 128);           \
-    else                                     \
-      MG_LAUNCH1(LR, MS, EX, 64);            \
+#define MG_LAUNCH(LR, MS, EX)                              \
+  do {                                                     \
+    if (sg.rows == 256 && sg.lpr == 2)                     \
+      MG_LAUNCH1(LR, MS, EX, 256, 2);                      \
+    else if (sg.rows == 256)                               \
+      MG_LAUNCH1(LR, MS, EX, 256, 1);                      \
+    else if (sg.rows == 128 && sg.lpr == 2)                \
+      MG_LAUNCH1(LR, MS, EX, 128, 2);                      \
+    else if (sg.rows == 128)                               \
+      MG_LAUNCH1(LR, MS, EX, 128, 1);                      \
+    else if (sg.lpr == 4)                                  \
+      MG_LAUNCH1(LR, MS, EX, 64, 4);                       \
+    else if (sg.lpr == 2)                                  \
+      MG_LAUNCH1(LR, MS, EX, 64, 2);                       \
+    else                                                   \
+      MG_LAUNCH1(LR, MS, EX, 64, 1);                       \
   } while (0)
 #define MG_LAUNCH2(LR)          \
   do {                          \

@@ -31,7 +31,8 @@ struct alignas(8) RecP1 {
 // entries in LDS; lrow is the local basis every incidence of these rows is relabelled to
 // (0 for vertex-dof rows, 4 for P2 edge-dof rows).
 struct RowSegment {
-  int32_t row_begin, row_end, lds_slots, lrow, rows;  // rows: rows (= lanes) per workgroup
+  int32_t row_begin, row_end, lds_slots, lrow;
+  int32_t rows, lpr;  // rows per workgroup, lanes per row
 };
 
 constexpr int kSpmvRows = 64;

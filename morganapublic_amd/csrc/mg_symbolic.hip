@@ -336,25 +336,34 @@ __global__ void k_inc_records(int64_t n_inc, const uint64_t *__restrict__ keys,
 // Assembly launches: the vertex-dof rows (relabelled local basis 0) and the
 // edge-dof rows (basis 4) of a P2 system go to separate launches; each launch
 // uses 64-row blocks and the LDS size of its largest block.
+static int env_int(const char *name, int dflt) {
+  const char *v = getenv(name);
+  return v ? atoi(v) : dflt;
+}
+
 static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvrows) {
   S.segs.clear();
-  const char *env = getenv("MGFE_EDGE_ROWS");
-  const int edge_rows = env ? atoi(env) : 64;
-  auto add = [&](int64_t a, int64_t b, int lrow, int rows) {
+  auto add = [&](int64_t a, int64_t b, int lrow, int rows, int lpr) {
     if (b <= a) return;
     for (;;) {
       int mx = 0;
       for (int64_t r = a; r < b; r += rows)
         mx = std::max(mx, rp[std::min<int64_t>(r + rows, b)] - rp[r]);
       if ((mx + 1) * 8 <= 64 * 1024 || rows == 64) {
-        S.segs.push_back(RowSegment{(int32_t)a, (int32_t)b, mx, lrow, rows});
+        S.segs.push_back(RowSegment{(int32_t)a, (int32_t)b, mx, lrow, rows, lpr});
         return;
       }
       rows /= 2;  // a block's entries must fit 64 KiB of LDS
     }
   };
-  add(0, nvrows, 0, 64);
-  add(nvrows, S.nrows, 4, (edge_rows == 128 || edge_rows == 256) ? edge_rows : 64);
+  auto pick = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
+  const int vl = env_int("MGFE_VERTEX_LPR", 4), er = env_int("MGFE_EDGE_ROWS", 128),
+            el = env_int("MGFE_EDGE_LPR", 2);
+  add(0, nvrows, 0, 64, pick(vl, 1, 2, 4) ? vl : 4);
+  const int erows = pick(er, 64, 128, 256) ? er : 128;
+  int elpr = (el == 1 || el == 2) ? el : 2;
+  if (erows == 64 && el == 4) elpr = 4;
+  add(nvrows, S.nrows, 4, erows, elpr);
 }
 
 void build_pattern(Ctx *c, int sys) {
