@@ -19,6 +19,7 @@
 // The workgroup then writes its contiguous CSR value range with coalesced
 // stores.
 #include <cstring>
+#include <type_traits>
 
 #include "mg_internal.h"
 
@@ -401,7 +402,9 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
         idx[m] = (pm == 0xFF || !valid) ? dummy : base + pm;
       }
       // the LPR lanes of a row add in turn (lane sub in phase sub), so every row
-      // receives its incidences in their stored order
+      // receives its incidences in their stored order.  The lanes of a row are
+      // in one wavefront, whose LDS operations execute in issue order; the
+      // compiler barrier keeps the phases from being merged into one update.
 #pragma unroll
       for (int ph = 0; ph < LPR; ++ph) {
         if (LPR == 1 || sub == ph) {
@@ -411,6 +414,7 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
 #pragma unroll
           for (int m = 0; m < NL; ++m) acc[idx[m]] = a[m] + v[m];
         }
+        if (LPR > 1) asm volatile("" ::: "memory");
       }
       A.r = load_rec<NL>(rec, at(j + 3 * LPR));
     };
@@ -439,6 +443,11 @@ static bool rule_exact(int order, int term, int rule) {
   return order == 1 ? rule >= 4 : rule >= 11;  // mass: degree 2 / degree 4
 }
 
+template <int V>
+using IC = std::integral_constant<int, V>;
+template <bool V>
+using BC = std::integral_constant<bool, V>;
+
 template <int NL, typename T, typename Rec>
 static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals) {
   const bool stiff = op->terms & MG_TERM_STIFF, mass = op->terms & MG_TERM_MASS;
@@ -452,48 +461,42 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals) {
     const int nblk = (sg.row_end - sg.row_begin + sg.rows - 1) / sg.rows;
     const size_t lds = sizeof(T) * (size_t)(sg.lds_slots + 1);
     const Rec *rec = (const Rec *)S.inc_rec;
-#define MG_LAUNCH1(LR, MS, EX, RW, LP)                                                         \
-  k_asm_rows<NL, LR, T, MS, EX, RW, LP, Rec><<<nblk, RW * LP, lds, c->stream>>>(               \
-      sg.row_begin, sg.row_end, sg.lds_slots, pp, S.rowptr, S.inc_ptr, rec, c->tets, c->xyz,   \
-      ksc, ksf, kmc, kmf, vals)
-#define MG_LAUNCH(LR, MS, EX)                              \
-  do {                                                     \
-    if (sg.rows == 256 && sg.lpr == 2)                     \
-      MG_LAUNCH1(LR, MS, EX, 256, 2);                      \
-    else if (sg.rows == 256)                               \
-      MG_LAUNCH1(LR, MS, EX, 256, 1);                      \
-    else if (sg.rows == 128 && sg.lpr == 2)                \
-      MG_LAUNCH1(LR, MS, EX, 128, 2);                      \
-    else if (sg.rows == 128)                               \
-      MG_LAUNCH1(LR, MS, EX, 128, 1);                      \
-    else if (sg.lpr == 4)                                  \
-      MG_LAUNCH1(LR, MS, EX, 64, 4);                       \
-    else if (sg.lpr == 2)                                  \
-      MG_LAUNCH1(LR, MS, EX, 64, 2);                       \
-    else                                                   \
-      MG_LAUNCH1(LR, MS, EX, 64, 1);                       \
-  } while (0)
-#define MG_LAUNCH2(LR)          \
-  do {                          \
-    if (exact) {                \
-      if (mass)                 \
-        MG_LAUNCH(LR, true, true); \
-      else                      \
-        MG_LAUNCH(LR, false, true); \
-    } else {                    \
-      if (mass)                 \
-        MG_LAUNCH(LR, true, false); \
-      else                      \
-        MG_LAUNCH(LR, false, false); \
-    }                           \
-  } while (0)
+    auto go = [&](auto lr, auto ms, auto ex, auto rw, auto lp) {
+      k_asm_rows<NL, decltype(lr)::value, T, decltype(ms)::value, decltype(ex)::value,
+                 decltype(rw)::value, decltype(lp)::value, Rec>
+          <<<nblk, decltype(rw)::value * decltype(lp)::value, lds, c->stream>>>(
+              sg.row_begin, sg.row_end, sg.lds_slots, pp, S.rowptr, S.inc_ptr, rec, c->tets,
+              c->xyz, ksc, ksf, kmc, kmf, vals);
+    };
+    // vertex-dof rows: 64 rows per workgroup, 1/2/4 lanes per row; P2 edge-dof
+    // rows: 64/128/256 rows per workgroup, 1/2 lanes per row
+    auto shape = [&](auto lr, auto ms, auto ex) {
+      if constexpr (decltype(lr)::value == 0) {
+        if (sg.lpr == 4)
+          go(lr, ms, ex, IC<64>{}, IC<4>{});
+        else if (sg.lpr == 2)
+          go(lr, ms, ex, IC<64>{}, IC<2>{});
+        else
+          go(lr, ms, ex, IC<64>{}, IC<1>{});
+      } else {
+        if (sg.rows == 256)
+          sg.lpr == 2 ? go(lr, ms, ex, IC<256>{}, IC<2>{}) : go(lr, ms, ex, IC<256>{}, IC<1>{});
+        else if (sg.rows == 128)
+          sg.lpr == 2 ? go(lr, ms, ex, IC<128>{}, IC<2>{}) : go(lr, ms, ex, IC<128>{}, IC<1>{});
+        else
+          sg.lpr == 2 ? go(lr, ms, ex, IC<64>{}, IC<2>{}) : go(lr, ms, ex, IC<64>{}, IC<1>{});
+      }
+    };
+    auto terms = [&](auto lr) {
+      if (exact)
+        mass ? shape(lr, BC<true>{}, BC<true>{}) : shape(lr, BC<false>{}, BC<true>{});
+      else
+        mass ? shape(lr, BC<true>{}, BC<false>{}) : shape(lr, BC<false>{}, BC<false>{});
+    };
     if (sg.lrow == 0)
-      MG_LAUNCH2(0);
-    else if (NL == 10)
-      MG_LAUNCH2(4);
-#undef MG_LAUNCH2
-#undef MG_LAUNCH
-#undef MG_LAUNCH1
+      terms(IC<0>{});
+    else if constexpr (NL == 10)
+      terms(IC<4>{});
     MG_HIP(hipGetLastError());
   }
 }

@@ -357,12 +357,13 @@ static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvr
     }
   };
   auto pick = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
-  const int vl = env_int("MGFE_VERTEX_LPR", 4), er = env_int("MGFE_EDGE_ROWS", 128),
+  // launch shapes (MGFE_VERTEX_LPR 1/2/4, MGFE_EDGE_ROWS 64/128/256,
+  // MGFE_EDGE_LPR 1/2) only change speed, never the result
+  const int vl = env_int("MGFE_VERTEX_LPR", 4), er = env_int("MGFE_EDGE_ROWS", 256),
             el = env_int("MGFE_EDGE_LPR", 2);
   add(0, nvrows, 0, 64, pick(vl, 1, 2, 4) ? vl : 4);
-  const int erows = pick(er, 64, 128, 256) ? er : 128;
-  int elpr = (el == 1 || el == 2) ? el : 2;
-  if (erows == 64 && el == 4) elpr = 4;
+  const int erows = pick(er, 64, 128, 256) ? er : 256;
+  const int elpr = (el == 1 || el == 2) ? el : 2;
   add(nvrows, S.nrows, 4, erows, elpr);
 }
 
