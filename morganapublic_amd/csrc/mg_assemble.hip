@@ -105,43 +105,137 @@ __device__ __forceinline__ GeomOut tet_geometry(const double *__restrict__ xyz, 
   return g;
 }
 
-// The 10 relabelling permutations packed 2 bits per entry, 8 bits per l:
-// byte l of (perm_hi:perm_lo) = perm[l][0] | perm[l][1] << 2 | ... (kernel
-// arguments, decoded with ALU ops so no memory access depends on l).
-struct PermPack {
-  uint64_t lo;
-  uint32_t hi;
+// ---- per-element factors ------------------------------------------------------
+// With n_j = det J grad(lambda_j) (n_1 = e2 x e3, n_2 = e3 x e1, n_3 = e1 x e2,
+// n_0 = -(n_1 + n_2 + n_3), e_k = P_k - P_0; det J signed as
+// geoMapInterface::getGradientDet, positive on a valid mesh), the record of an
+// element holds, for its 6 edges (a, b) in kEdgeV order,
+//   S_ab = k / (6 det) n_a . n_b  =  k vol grad(lambda_a) . grad(lambda_b)
+// and, with a mass term, c_mass det.  Since sum_b grad(lambda_b) = 0 the
+// diagonal follows as S_aa = -sum_{b != a} S_ab.  Every stiffness and mass row
+// of the element is a fixed linear map of these 6 (7) numbers, so the row
+// kernels read one small record per incidence instead of 4 vertex coordinates.
+// Layout: FP64 records of 6 (stiffness) or 8 (with mass: 7 used) doubles, FP32
+// records of 8 floats.
+template <typename T, bool MASS>
+struct Geo {
+  static constexpr int N = (sizeof(T) == 8 && !MASS) ? 6 : 8;  // stride in values
+  T s[N];
 };
 
-static PermPack pack_perm() {
+template <typename T, bool MASS>
+__device__ __forceinline__ Geo<T, MASS> load_geo(const T *__restrict__ geo, int32_t e) {
+  Geo<T, MASS> g;
+  constexpr int N = Geo<T, MASS>::N;
+  if constexpr (sizeof(T) == 8) {
+    const double2 *p = reinterpret_cast<const double2 *>(geo + (size_t)e * N);
+#pragma unroll
+    for (int k = 0; k < (MASS ? 4 : 3); ++k) {
+      const double2 q = p[k];
+      g.s[2 * k] = q.x;
+      g.s[2 * k + 1] = q.y;
+    }
+  } else {
+    const float4 *p = reinterpret_cast<const float4 *>(geo + (size_t)e * N);
+    const float4 q = p[0];
+    g.s[0] = q.x;
+    g.s[1] = q.y;
+    g.s[2] = q.z;
+    g.s[3] = q.w;
+    if (MASS) {
+      const float4 q1 = p[1];
+      g.s[4] = q1.x;
+      g.s[5] = q1.y;
+      g.s[6] = q1.z;
+      g.s[7] = q1.w;
+    } else {
+      const float2 q1 = *reinterpret_cast<const float2 *>(p + 1);
+      g.s[4] = q1.x;
+      g.s[5] = q1.y;
+    }
+  }
+  return g;
+}
+
+template <typename T, bool MASS>
+__global__ __launch_bounds__(256) void k_elem_factors(
+    int64_t nT, const int32_t *__restrict__ tets, const double *__restrict__ xyz,
+    double ks_const, const double *__restrict__ ks_field, double km_const,
+    const double *__restrict__ km_field, T *__restrict__ geo) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nT) return;
+  const int4 t = reinterpret_cast<const int4 *>(tets)[e];
+  const int id[4] = {t.x, t.y, t.z, t.w};
+  double P[4][3];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) P[v][r] = xyz[3 * (size_t)id[v] + r];
+  double d[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) d[k][r] = P[k + 1][r] - P[0][r];
+  double n[4][3];
+  n[1][0] = d[1][1] * d[2][2] - d[1][2] * d[2][1];
+  n[1][1] = d[1][2] * d[2][0] - d[1][0] * d[2][2];
+  n[1][2] = d[1][0] * d[2][1] - d[1][1] * d[2][0];
+  n[2][0] = d[2][1] * d[0][2] - d[2][2] * d[0][1];
+  n[2][1] = d[2][2] * d[0][0] - d[2][0] * d[0][2];
+  n[2][2] = d[2][0] * d[0][1] - d[2][1] * d[0][0];
+  n[3][0] = d[0][1] * d[1][2] - d[0][2] * d[1][1];
+  n[3][1] = d[0][2] * d[1][0] - d[0][0] * d[1][2];
+  n[3][2] = d[0][0] * d[1][1] - d[0][1] * d[1][0];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) n[0][r] = -(n[1][r] + n[2][r] + n[3][r]);
+  const double det = d[0][0] * n[1][0] + d[0][1] * n[1][1] + d[0][2] * n[1][2];
+  const double ks = ks_field ? ks_field[e] : ks_const;
+  const double sc = ks / (6.0 * det);
+  constexpr int EA[6] = {0, 1, 2, 0, 1, 2}, EB[6] = {1, 2, 0, 3, 3, 3};  // kEdgeV
+  constexpr int N = Geo<T, MASS>::N;
+  T out[8];
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+    out[j] = (T)(sc * (n[EA[j]][0] * n[EB[j]][0] + n[EA[j]][1] * n[EB[j]][1] +
+                       n[EA[j]][2] * n[EB[j]][2]));
+  out[6] = MASS ? (T)((km_field ? km_field[e] : km_const) * det) : (T)0;
+  out[7] = (T)0;
+  if constexpr (sizeof(T) == 8) {
+    double2 *q = reinterpret_cast<double2 *>(geo + (size_t)e * N);
+#pragma unroll
+    for (int k = 0; k < N / 2; ++k) q[k] = make_double2(out[2 * k], out[2 * k + 1]);
+  } else {
+    float4 *q = reinterpret_cast<float4 *>(geo + (size_t)e * N);
+    q[0] = make_float4(out[0], out[1], out[2], out[3]);
+    q[1] = make_float4(out[4], out[5], out[6], out[7]);
+  }
+}
+
+// The element of an incidence is relabelled by the even vertex permutation
+// that moves the row's basis to local index 0 (vertex) or 4 (edge (0,1)); the
+// relabelled edge j is original edge sigma_l(j).  EdgePack.w[l] holds
+// sigma_l(j) in bits 3j..3j+2 (kernel arguments; selected with ALU ops, so no
+// memory access depends on l).
+struct EdgePack {
+  uint32_t w[10];
+};
+
+static EdgePack pack_edges() {
   int perm[10][4], sigma[10][10];
   relabel_tables(perm, sigma);
-  PermPack p{0, 0};
-  for (int l = 0; l < 10; ++l) {
-    uint64_t b = perm[l][0] | perm[l][1] << 2 | perm[l][2] << 4 | perm[l][3] << 6;
-    if (l < 8)
-      p.lo |= b << (8 * l);
-    else
-      p.hi |= (uint32_t)(b << (8 * (l - 8)));
-  }
+  EdgePack p{};
+  for (int l = 0; l < 10; ++l)
+    for (int j = 0; j < 6; ++j) p.w[l] |= (uint32_t)(sigma[l][4 + j] - 4) << (3 * j);
   return p;
 }
 
-__device__ __forceinline__ int4 load_tet(const int32_t *__restrict__ tets, int32_t el) {
-  return *reinterpret_cast<const int4 *>(tets + 4 * (size_t)(el >> 4));
-}
-
-// the 4 vertex ids of the element relabelled for row basis l = el & 15
-__device__ __forceinline__ void relabel(int4 t4, int32_t el, PermPack pp, int32_t id[4]) {
-  const int l = el & 15;
-  const uint32_t byte = l < 8 ? (uint32_t)(pp.lo >> (8 * l)) : (pp.hi >> (8 * (l - 8)));
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const uint32_t p = (byte >> (2 * k)) & 3u;
-    const int a = (p & 1u) ? t4.y : t4.x;
-    const int b = (p & 1u) ? t4.w : t4.z;
-    id[k] = (p & 2u) ? b : a;
-  }
+template <typename T>
+__device__ __forceinline__ T pick6(const T *s, uint32_t k) {
+  const T a = (k & 1u) ? s[1] : s[0];
+  const T b = (k & 1u) ? s[3] : s[2];
+  const T c = (k & 1u) ? s[5] : s[4];
+  const T ab = (k & 2u) ? b : a;
+  return (k & 4u) ? c : ab;
 }
 
 // An incidence record read as whole dwords: el, then the pos bytes.
@@ -181,68 +275,20 @@ __device__ __forceinline__ RecWords<4> load_rec<4>(const void *__restrict__ rec,
   return r;
 }
 
-struct Coords {
-  double x[12];
-  double ks, km;
-};
-
-__device__ __forceinline__ Coords load_coords(const double *__restrict__ xyz, const int32_t id[4],
-                                              int64_t e, double ks_const,
-                                              const double *__restrict__ ks_field, double km_const,
-                                              const double *__restrict__ km_field) {
-  Coords c;
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    c.x[3 * v] = xyz[3 * (size_t)id[v]];
-    c.x[3 * v + 1] = xyz[3 * (size_t)id[v] + 1];
-    c.x[3 * v + 2] = xyz[3 * (size_t)id[v] + 2];
-  }
-  c.ks = ks_field ? ks_field[e] : ks_const;
-  c.km = km_field ? km_field[e] : km_const;
-  return c;
-}
-
-// c det (J^{-1} J^{-T})_k for the 6 symmetric pairs, and c_mass det, of the
-// (relabelled) element: with adj = adjugate of J, det J^{-1} J^{-T} = adj adj^T / det.
-__device__ __forceinline__ void elem_factors(const Coords &c, double g[7]) {
-  double T[3][3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-#pragma unroll
-    for (int r = 0; r < 3; ++r) T[r][k] = c.x[3 * (k + 1) + r] - c.x[r];
-  double adj[3][3];
-  adj[0][0] = T[1][1] * T[2][2] - T[1][2] * T[2][1];
-  adj[0][1] = -(T[0][1] * T[2][2] - T[0][2] * T[2][1]);
-  adj[0][2] = T[0][1] * T[1][2] - T[0][2] * T[1][1];
-  adj[1][0] = -(T[1][0] * T[2][2] - T[1][2] * T[2][0]);
-  adj[1][1] = T[0][0] * T[2][2] - T[0][2] * T[2][0];
-  adj[1][2] = -(T[0][0] * T[1][2] - T[0][2] * T[1][0]);
-  adj[2][0] = T[1][0] * T[2][1] - T[1][1] * T[2][0];
-  adj[2][1] = -(T[0][0] * T[2][1] - T[0][1] * T[2][0]);
-  adj[2][2] = T[0][0] * T[1][1] - T[0][1] * T[1][0];
-  const double det = T[0][0] * adj[0][0] + T[0][1] * adj[1][0] + T[0][2] * adj[2][0];
-  const double s = c.ks / det;
-  const int KA[6] = {0, 1, 2, 0, 0, 1}, KB[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-  for (int k = 0; k < 6; ++k)
-    g[k] = s * (adj[KA[k]][0] * adj[KB[k]][0] + adj[KA[k]][1] * adj[KB[k]][1] +
-                adj[KA[k]][2] * adj[KB[k]][2]);
-  g[6] = c.km * det;
-}
-
 // ---- closed form of the stiffness row for quadrature rules exact on it ------
-// With n_j = det J grad(lambda_j) (n_1 = e2 x e3, n_2 = e3 x e1, n_3 = e1 x e2,
-// n_0 = -(n_1 + n_2 + n_3), e_k = P_k - P_0) and D_pq = n_p . n_q, the
-// stiffness entry is k/(6 det) * Q_lm(D), where Q integrates the products of
-// the basis gradients with  int lambda_p = vol/4  and
-// int lambda_p lambda_q = vol (1 + delta_pq) / 20:
+// With D_pq = k vol grad(lambda_p) . grad(lambda_q) of the relabelled element,
+// the stiffness entry is Q_lm(D), where Q integrates the products of the basis
+// gradients with  int lambda_p = vol/4  and  int lambda_p lambda_q = vol (1 + delta_pq) / 20:
 //   P1:                  Q_ij = D_ij
 //   P2 vertex i, vertex j: D_ij ((4/5)(1 + delta_ij) - 1)
 //   P2 vertex i, edge ab:  4 (D_ib E(i,a) + D_ia E(i,b)),  E(i,a) = (1 + delta_ia)/5 - 1/4
 //   P2 edge ab, edge cd:   16 (D_bd F(a,c) + D_bc F(a,d) + D_ad F(b,c) + D_ac F(b,d)),
 //                          F(p,q) = (1 + delta_pq)/20
 // This is what any quadrature rule exact on the integrand (P1: all rules;
-// P2: 4, 11, 15 points) returns, up to rounding.
+// P2: 4, 11, 15 points) returns, up to rounding.  Otherwise the row is
+//   sum_k g_k tab[k][LROW][m],  g_(ab) = c det (J^{-1} J^{-T})_ab = 6 D_(a+1)(b+1),
+// with the quadrature of integratorOEL3d_STD contracted once on the reference
+// element (tab, constant memory, compile-time indices).
 __host__ __device__ constexpr int edge_a(int j) {
   return j == 0 ? 0 : j == 1 ? 1 : j == 2 ? 2 : j == 3 ? 0 : j == 4 ? 1 : 2;
 }
@@ -253,97 +299,91 @@ __host__ __device__ constexpr double dlt(int p, int q) { return p == q ? 1.0 : 0
 __host__ __device__ constexpr double coefE(int i, int a) { return (1.0 + dlt(i, a)) / 5.0 - 0.25; }
 __host__ __device__ constexpr double coefF(int p, int q) { return (1.0 + dlt(p, q)) / 20.0; }
 
-// D[p][q] for p in {0, 1}, q in 0..3 of the relabelled element, and det
-__device__ __forceinline__ void bary_dots(const Coords &c, bool need_row1, double D0[4],
-                                          double D1[4], double &det) {
-  double e[3][3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-#pragma unroll
-    for (int r = 0; r < 3; ++r) e[k][r] = c.x[3 * (k + 1) + r] - c.x[r];
-  double n[4][3];
-  n[1][0] = e[1][1] * e[2][2] - e[1][2] * e[2][1];
-  n[1][1] = e[1][2] * e[2][0] - e[1][0] * e[2][2];
-  n[1][2] = e[1][0] * e[2][1] - e[1][1] * e[2][0];
-  n[2][0] = e[2][1] * e[0][2] - e[2][2] * e[0][1];
-  n[2][1] = e[2][2] * e[0][0] - e[2][0] * e[0][2];
-  n[2][2] = e[2][0] * e[0][1] - e[2][1] * e[0][0];
-  n[3][0] = e[0][1] * e[1][2] - e[0][2] * e[1][1];
-  n[3][1] = e[0][2] * e[1][0] - e[0][0] * e[1][2];
-  n[3][2] = e[0][0] * e[1][1] - e[0][1] * e[1][0];
-#pragma unroll
-  for (int r = 0; r < 3; ++r) n[0][r] = -(n[1][r] + n[2][r] + n[3][r]);
-  det = e[0][0] * n[1][0] + e[0][1] * n[1][1] + e[0][2] * n[1][2];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) D0[q] = n[0][0] * n[q][0] + n[0][1] * n[q][1] + n[0][2] * n[q][2];
-  if (need_row1) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) D1[q] = n[1][0] * n[q][0] + n[1][1] * n[q][1] + n[1][2] * n[q][2];
+// Row LROW (0: vertex, 4: edge (0,1)) of the relabelled element matrix of
+// incidence basis l, from the element record.
+template <int NL, int LROW, typename T, bool MASS, bool EXACT>
+__device__ __forceinline__ void row_values(const Geo<T, MASS> &g, int l, const EdgePack &ep,
+                                           T v[NL]) {
+  uint32_t w;
+  if (LROW == 0) {
+    const uint32_t a = (l & 1) ? ep.w[1] : ep.w[0], b = (l & 1) ? ep.w[3] : ep.w[2];
+    w = (l & 2) ? b : a;
+  } else {
+    w = pick6(ep.w + 4, (uint32_t)(l - 4));
   }
-}
-
-// Row LROW (0: vertex, 4: edge (0,1)) of the relabelled element matrix, exact.
-template <int NL, int LROW, bool MASS, typename T>
-__device__ __forceinline__ void exact_row(const Coords &c, T v[NL]) {
-  double D0[4], D1[4], det;
-  bary_dots(c, LROW == 4, D0, D1, det);
-  const double s = c.ks / (6.0 * det);
-  const double md = c.km * det;
+  // relabelled edges (kEdgeV order): e0 = D01, e1 = D12, e2 = D02, e3 = D03, e4 = D13, e5 = D23
+  T e[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) e[j] = pick6(g.s, (w >> (3 * j)) & 7u);
+  const T D[4][4] = {{-(e[0] + e[2] + e[3]), e[0], e[2], e[3]},
+                     {e[0], -(e[0] + e[1] + e[4]), e[1], e[4]},
+                     {e[2], e[1], -(e[1] + e[2] + e[5]), e[5]},
+                     {e[3], e[4], e[5], -(e[3] + e[4] + e[5])}};
+  const T md = MASS ? g.s[6] : (T)0;
+  if (!EXACT) {
+    const T gk[6] = {(T)6 * D[1][1], (T)6 * D[2][2], (T)6 * D[3][3],
+                     (T)6 * D[1][2], (T)6 * D[1][3], (T)6 * D[2][3]};
+#pragma unroll
+    for (int m = 0; m < NL; ++m) {
+      T sum = gk[0] * tab<T>((0 * NL + LROW) * NL + m);
+#pragma unroll
+      for (int k = 1; k < 6; ++k) sum += gk[k] * tab<T>((k * NL + LROW) * NL + m);
+      if (MASS) sum += md * tab<T>((6 * NL + LROW) * NL + m);
+      v[m] = sum;
+    }
+    return;
+  }
   if (NL == 4) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      double val = s * D0[j];
-      if (MASS) val += md * tab<double>((6 * NL + LROW) * NL + j);
-      v[j] = (T)val;
+      T val = D[0][j];
+      if (MASS) val += md * tab<T>((6 * NL + LROW) * NL + j);
+      v[j] = val;
     }
     return;
   }
 #pragma unroll
   for (int m = 0; m < NL; ++m) {
-    double q;
+    T q;
     if (LROW == 0) {
       if (m < 4) {
-        q = D0[m] * (0.8 * (1.0 + dlt(0, m)) - 1.0);
+        q = D[0][m] * (T)(0.8 * (1.0 + dlt(0, m)) - 1.0);
       } else {
         const int a = edge_a(m - 4), b = edge_b(m - 4);
-        q = 4.0 * (D0[b] * coefE(0, a) + D0[a] * coefE(0, b));
+        q = (T)4 * (D[0][b] * (T)coefE(0, a) + D[0][a] * (T)coefE(0, b));
       }
     } else {
       if (m < 4) {
-        q = 4.0 * (D1[m] * coefE(m, 0) + D0[m] * coefE(m, 1));
+        q = (T)4 * (D[1][m] * (T)coefE(m, 0) + D[0][m] * (T)coefE(m, 1));
       } else {
         const int cc = edge_a(m - 4), d = edge_b(m - 4);
         // row edge (a,b) = (0,1): D_bd F(a,c) + D_bc F(a,d) + D_ad F(b,c) + D_ac F(b,d)
-        q = 16.0 * (D1[d] * coefF(0, cc) + D1[cc] * coefF(0, d) + D0[d] * coefF(1, cc) +
-                    D0[cc] * coefF(1, d));
+        q = (T)16 * (D[1][d] * (T)coefF(0, cc) + D[1][cc] * (T)coefF(0, d) +
+                     D[0][d] * (T)coefF(1, cc) + D[0][cc] * (T)coefF(1, d));
       }
     }
-    double val = s * q;
-    if (MASS) val += md * tab<double>((6 * NL + LROW) * NL + m);
-    v[m] = (T)val;
+    if (MASS) q += md * tab<T>((6 * NL + LROW) * NL + m);
+    v[m] = q;
   }
 }
 
-// One lane per CSR row, 64 rows per workgroup, the block's entries staged in
-// LDS.  A lane walks its row's incidences in their fixed order; each element
-// is relabelled so the row is its local basis LROW, and row LROW of the element
-// matrix is sum_k g_k tab[k][LROW][m] with compile-time table indices (scalar
-// loads).  Software pipeline over the incidences: while incidence j is
-// contracted, the coordinates of j+1, the vertex ids of j+2 and the record of
-// j+3 are in flight.
+// ROWS rows per workgroup, LPR lanes per row, the block's CSR entries staged
+// in LDS.  The lanes of a row take its incidences in turn (lane sub takes
+// sub, sub + LPR, ...) and add their rows of the element matrices in LPR
+// ordered phases, so every entry sums its contributions in incidence order,
+// independent of ROWS and LPR.  Software pipeline over a lane's incidences:
+// while incidence j is contracted, the element record of j+1 and the
+// incidence record of j+3 are in flight.
 template <int NL, int LROW, typename T, bool MASS, bool EXACT, int ROWS, int LPR, typename Rec>
 __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
-    int32_t row_begin, int32_t row_end, int32_t dummy, PermPack pp,
-    const int32_t *__restrict__ rowptr,
-    const int32_t *__restrict__ inc_ptr, const Rec *__restrict__ rec,
-    const int32_t *__restrict__ tets, const double *__restrict__ xyz, double ks_const,
-    const double *__restrict__ ks_field, double km_const, const double *__restrict__ km_field,
-    T *__restrict__ vals) {
+    int32_t row_begin, int32_t row_end, int32_t dummy, EdgePack ep,
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ inc_ptr,
+    const Rec *__restrict__ rec, const T *__restrict__ geo, T *__restrict__ vals) {
   extern __shared__ __align__(16) char lds_raw[];
   T *acc = reinterpret_cast<T *>(lds_raw);
   constexpr int NT = ROWS * LPR;
   const int lane = threadIdx.x;
-  const int sub = lane % LPR;  // LPR lanes share a row and take every LPR-th incidence
+  const int sub = lane % LPR;
   const int r0 = row_begin + xcd_chunk(blockIdx.x, gridDim.x) * ROWS;
   const int r1 = min(r0 + ROWS, row_end);
   const int s0 = rowptr[r0];
@@ -355,44 +395,22 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
     const int base = rowptr[r] - s0;
     const int iend = inc_ptr[r + 1];
     int i = inc_ptr[r] + sub;
-    // Three slots rotate through the stages (record -> vertex ids ->
-    // coordinates -> contraction); each step issues the next stage of the two
-    // following incidences and the record three ahead, then contracts the
-    // current one.  Unrolled by three so every load lands in the slot that
-    // consumes it and no loaded register is copied; the body has no branch:
-    // indices past the row's last incidence are clamped (loads stay in
-    // bounds) and their contribution goes to the dummy slot with value 0.
+    // Three slots rotate; unrolled by three so every load lands in the slot
+    // that consumes it and no loaded register is copied.  The body has no
+    // branch: indices past the row's last incidence are clamped (loads stay
+    // in bounds) and their contribution goes to the dummy slot with value 0.
     const int last = iend > i - sub ? iend - 1 : 0;  // a row without incidences reads record 0
     auto at = [&](int j) { return j < iend ? j : last; };
     struct Slot {
       RecWords<NL> r;
-      int4 t;
-      Coords c;
+      Geo<T, MASS> g;
     };
     Slot S0, S1, S2;
-    auto step = [&](Slot &A, Slot &B, Slot &C, int j) {
-      {
-        int32_t id[4];
-        relabel(B.t, B.r.el, pp, id);
-        B.c = load_coords(xyz, id, B.r.el >> 4, ks_const, ks_field, km_const, km_field);
-      }
-      C.t = load_tet(tets, C.r.el);
+    auto step = [&](Slot &A, Slot &B, int j) {
+      B.g = load_geo<T, MASS>(geo, B.r.el >> 4);
       const bool valid = j < iend;
       T v[NL];
-      if (EXACT) {
-        exact_row<NL, LROW, MASS, T>(A.c, v);
-      } else {
-        double g[7];
-        elem_factors(A.c, g);
-#pragma unroll
-        for (int m = 0; m < NL; ++m) {
-          T sum = (T)g[0] * tab<T>((0 * NL + LROW) * NL + m);
-#pragma unroll
-          for (int k = 1; k < 6; ++k) sum += (T)g[k] * tab<T>((k * NL + LROW) * NL + m);
-          if (MASS) sum += (T)g[6] * tab<T>((6 * NL + LROW) * NL + m);
-          v[m] = sum;
-        }
-      }
+      row_values<NL, LROW, T, MASS, EXACT>(A.g, A.r.el & 15, ep, v);
 #pragma unroll
       for (int m = 0; m < NL; ++m) v[m] = valid ? v[m] : (T)0;
       int idx[NL];
@@ -401,10 +419,9 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
         const int pm = A.r.pos(m);
         idx[m] = (pm == 0xFF || !valid) ? dummy : base + pm;
       }
-      // the LPR lanes of a row add in turn (lane sub in phase sub), so every row
-      // receives its incidences in their stored order.  The lanes of a row are
-      // in one wavefront, whose LDS operations execute in issue order; the
-      // compiler barrier keeps the phases from being merged into one update.
+      // The lanes of a row are in one wavefront, whose LDS operations execute
+      // in issue order; the compiler barrier keeps the phases from being
+      // merged into one update.
 #pragma unroll
       for (int ph = 0; ph < LPR; ++ph) {
         if (LPR == 1 || sub == ph) {
@@ -419,18 +436,13 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
       A.r = load_rec<NL>(rec, at(j + 3 * LPR));
     };
     S0.r = load_rec<NL>(rec, at(i));
-    {
-      int32_t id[4];
-      relabel(load_tet(tets, S0.r.el), S0.r.el, pp, id);
-      S0.c = load_coords(xyz, id, S0.r.el >> 4, ks_const, ks_field, km_const, km_field);
-    }
     S1.r = load_rec<NL>(rec, at(i + LPR));
-    S1.t = load_tet(tets, S1.r.el);
     S2.r = load_rec<NL>(rec, at(i + 2 * LPR));
+    S0.g = load_geo<T, MASS>(geo, S0.r.el >> 4);
     for (; i < iend; i += 3 * LPR) {
-      step(S0, S1, S2, i);
-      step(S1, S2, S0, i + LPR);
-      step(S2, S0, S1, i + 2 * LPR);
+      step(S0, S1, i);
+      step(S1, S2, i + LPR);
+      step(S2, S0, i + 2 * LPR);
     }
   }
   __syncthreads();
@@ -456,17 +468,34 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals) {
   const double *kmf = (mass && op->mass_coeff == MG_COEFF_P0) ? c->field[MG_FIELD_MASS] : nullptr;
   const bool exact = (!stiff || rule_exact(c->order, MG_TERM_STIFF, op->stiff_rule)) &&
                      (!mass || rule_exact(c->order, MG_TERM_MASS, op->mass_rule));
-  static const PermPack pp = pack_perm();
+  static const EdgePack ep = pack_edges();
+  // element records
+  const int stride = (sizeof(T) == 8 && !mass) ? 6 : 8;
+  const size_t gbytes = sizeof(T) * stride * (size_t)c->nT;
+  if (c->geom_bytes < gbytes) {
+    dfree(c, c->geom);
+    c->geom = dalloc<char>(c, gbytes);
+    c->geom_bytes = gbytes;
+  }
+  T *geo = (T *)c->geom;
+  if (c->nT > 0) {
+    if (mass)
+      k_elem_factors<T, true><<<blocks_for(c->nT, 256), 256, 0, c->stream>>>(
+          c->nT, c->tets, c->xyz, ksc, ksf, kmc, kmf, geo);
+    else
+      k_elem_factors<T, false><<<blocks_for(c->nT, 256), 256, 0, c->stream>>>(
+          c->nT, c->tets, c->xyz, ksc, ksf, kmc, kmf, geo);
+    MG_HIP(hipGetLastError());
+  }
+  const Rec *rec = (const Rec *)S.inc_rec;
   for (const RowSegment &sg : S.segs) {
     const int nblk = (sg.row_end - sg.row_begin + sg.rows - 1) / sg.rows;
     const size_t lds = sizeof(T) * (size_t)(sg.lds_slots + 1);
-    const Rec *rec = (const Rec *)S.inc_rec;
     auto go = [&](auto lr, auto ms, auto ex, auto rw, auto lp) {
       k_asm_rows<NL, decltype(lr)::value, T, decltype(ms)::value, decltype(ex)::value,
                  decltype(rw)::value, decltype(lp)::value, Rec>
           <<<nblk, decltype(rw)::value * decltype(lp)::value, lds, c->stream>>>(
-              sg.row_begin, sg.row_end, sg.lds_slots, pp, S.rowptr, S.inc_ptr, rec, c->tets,
-              c->xyz, ksc, ksf, kmc, kmf, vals);
+              sg.row_begin, sg.row_end, sg.lds_slots, ep, S.rowptr, S.inc_ptr, rec, geo, vals);
     };
     // vertex-dof rows: 64 rows per workgroup, 1/2/4 lanes per row; P2 edge-dof
     // rows: 64/128/256 rows per workgroup, 1/2 lanes per row
