@@ -304,6 +304,9 @@ __host__ __device__ constexpr double coefF(int p, int q) { return (1.0 + dlt(p, 
 template <int NL, int LROW, typename T, bool MASS, bool EXACT>
 __device__ __forceinline__ void row_values(const Geo<T, MASS> &g, int l, const EdgePack &ep,
                                            T v[NL]) {
+  // every kernel shape must round alike: no fused multiply-add chosen per
+  // instantiation
+#pragma clang fp contract(off)
   uint32_t w;
   if (LROW == 0) {
     const uint32_t a = (l & 1) ? ep.w[1] : ep.w[0], b = (l & 1) ? ep.w[3] : ep.w[2];
