@@ -118,43 +118,8 @@ __device__ __forceinline__ GeomOut tet_geometry(const double *__restrict__ xyz, 
 // Layout: FP64 records of 6 (stiffness) or 8 (with mass: 7 used) doubles, FP32
 // records of 8 floats.
 template <typename T, bool MASS>
-struct Geo {
-  static constexpr int N = (sizeof(T) == 8 && !MASS) ? 6 : 8;  // stride in values
-  T s[N];
-};
-
-template <typename T, bool MASS>
-__device__ __forceinline__ Geo<T, MASS> load_geo(const T *__restrict__ geo, int32_t e) {
-  Geo<T, MASS> g;
-  constexpr int N = Geo<T, MASS>::N;
-  if constexpr (sizeof(T) == 8) {
-    const double2 *p = reinterpret_cast<const double2 *>(geo + (size_t)e * N);
-#pragma unroll
-    for (int k = 0; k < (MASS ? 4 : 3); ++k) {
-      const double2 q = p[k];
-      g.s[2 * k] = q.x;
-      g.s[2 * k + 1] = q.y;
-    }
-  } else {
-    const float4 *p = reinterpret_cast<const float4 *>(geo + (size_t)e * N);
-    const float4 q = p[0];
-    g.s[0] = q.x;
-    g.s[1] = q.y;
-    g.s[2] = q.z;
-    g.s[3] = q.w;
-    if (MASS) {
-      const float4 q1 = p[1];
-      g.s[4] = q1.x;
-      g.s[5] = q1.y;
-      g.s[6] = q1.z;
-      g.s[7] = q1.w;
-    } else {
-      const float2 q1 = *reinterpret_cast<const float2 *>(p + 1);
-      g.s[4] = q1.x;
-      g.s[5] = q1.y;
-    }
-  }
-  return g;
+constexpr int geo_stride() {
+  return (sizeof(T) == 8 && !MASS) ? 6 : 8;
 }
 
 template <typename T, bool MASS>
@@ -192,7 +157,7 @@ __global__ __launch_bounds__(256) void k_elem_factors(
   const double ks = ks_field ? ks_field[e] : ks_const;
   const double sc = ks / (6.0 * det);
   constexpr int EA[6] = {0, 1, 2, 0, 1, 2}, EB[6] = {1, 2, 0, 3, 3, 3};  // kEdgeV
-  constexpr int N = Geo<T, MASS>::N;
+  constexpr int N = geo_stride<T, MASS>();
   T out[8];
 #pragma unroll
   for (int j = 0; j < 6; ++j)
@@ -299,50 +264,42 @@ __host__ __device__ constexpr double dlt(int p, int q) { return p == q ? 1.0 : 0
 __host__ __device__ constexpr double coefE(int i, int a) { return (1.0 + dlt(i, a)) / 5.0 - 0.25; }
 __host__ __device__ constexpr double coefF(int p, int q) { return (1.0 + dlt(p, q)) / 20.0; }
 
-// Row LROW (0: vertex, 4: edge (0,1)) of the relabelled element matrix of
-// incidence basis l, from the element record.
+// Relabelled edges a row needs: vertex rows (0,1), (2,0), (0,3); edge rows
+// all but (2,3); the quadrature-table form all six.
+template <int LROW, bool EXACT>
+__host__ __device__ constexpr bool edge_needed(int j) {
+  return !EXACT || (LROW == 0 ? (j == 0 || j == 2 || j == 3) : j != 5);
+}
+
+// Row LROW (0: vertex, 4: edge (0,1)) of the relabelled element matrix from
+// the relabelled edge values e (kEdgeV order: e0 = D01, e1 = D12, e2 = D02,
+// e3 = D03, e4 = D13, e5 = D23) and md = c_mass det.
 template <int NL, int LROW, typename T, bool MASS, bool EXACT>
-__device__ __forceinline__ void row_values(const Geo<T, MASS> &g, int l, const EdgePack &ep,
-                                           T v[NL]) {
-  // every kernel shape must round alike: no fused multiply-add chosen per
-  // instantiation
+__device__ __forceinline__ void row_values(const T e[6], T md, T v[NL]) {
+  // every kernel shape must round alike: explicit fma only, no contraction
+  // chosen per instantiation
 #pragma clang fp contract(off)
-  uint32_t w;
-  if (LROW == 0) {
-    const uint32_t a = (l & 1) ? ep.w[1] : ep.w[0], b = (l & 1) ? ep.w[3] : ep.w[2];
-    w = (l & 2) ? b : a;
-  } else {
-    w = pick6(ep.w + 4, (uint32_t)(l - 4));
-  }
-  // relabelled edges (kEdgeV order): e0 = D01, e1 = D12, e2 = D02, e3 = D03, e4 = D13, e5 = D23
-  T e[6];
-#pragma unroll
-  for (int j = 0; j < 6; ++j) e[j] = pick6(g.s, (w >> (3 * j)) & 7u);
   const T D[4][4] = {{-(e[0] + e[2] + e[3]), e[0], e[2], e[3]},
                      {e[0], -(e[0] + e[1] + e[4]), e[1], e[4]},
                      {e[2], e[1], -(e[1] + e[2] + e[5]), e[5]},
                      {e[3], e[4], e[5], -(e[3] + e[4] + e[5])}};
-  const T md = MASS ? g.s[6] : (T)0;
   if (!EXACT) {
     const T gk[6] = {(T)6 * D[1][1], (T)6 * D[2][2], (T)6 * D[3][3],
-                     (T)6 * D[1][2], (T)6 * D[1][3], (T)6 * D[2][3]};
+                     (T)6 * e[1],    (T)6 * e[4],    (T)6 * e[5]};
 #pragma unroll
     for (int m = 0; m < NL; ++m) {
       T sum = gk[0] * tab<T>((0 * NL + LROW) * NL + m);
 #pragma unroll
-      for (int k = 1; k < 6; ++k) sum += gk[k] * tab<T>((k * NL + LROW) * NL + m);
-      if (MASS) sum += md * tab<T>((6 * NL + LROW) * NL + m);
+      for (int k = 1; k < 6; ++k) sum = fma(gk[k], tab<T>((k * NL + LROW) * NL + m), sum);
+      if (MASS) sum = fma(md, tab<T>((6 * NL + LROW) * NL + m), sum);
       v[m] = sum;
     }
     return;
   }
   if (NL == 4) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      T val = D[0][j];
-      if (MASS) val += md * tab<T>((6 * NL + LROW) * NL + j);
-      v[j] = val;
-    }
+    for (int j = 0; j < 4; ++j)
+      v[j] = MASS ? fma(md, tab<T>((6 * NL + LROW) * NL + j), D[0][j]) : D[0][j];
     return;
   }
 #pragma unroll
@@ -353,31 +310,36 @@ __device__ __forceinline__ void row_values(const Geo<T, MASS> &g, int l, const E
         q = D[0][m] * (T)(0.8 * (1.0 + dlt(0, m)) - 1.0);
       } else {
         const int a = edge_a(m - 4), b = edge_b(m - 4);
-        q = (T)4 * (D[0][b] * (T)coefE(0, a) + D[0][a] * (T)coefE(0, b));
+        q = fma(D[0][b], (T)(4.0 * coefE(0, a)), D[0][a] * (T)(4.0 * coefE(0, b)));
       }
     } else {
       if (m < 4) {
-        q = (T)4 * (D[1][m] * (T)coefE(m, 0) + D[0][m] * (T)coefE(m, 1));
+        q = fma(D[1][m], (T)(4.0 * coefE(m, 0)), D[0][m] * (T)(4.0 * coefE(m, 1)));
       } else {
         const int cc = edge_a(m - 4), d = edge_b(m - 4);
         // row edge (a,b) = (0,1): D_bd F(a,c) + D_bc F(a,d) + D_ad F(b,c) + D_ac F(b,d)
-        q = (T)16 * (D[1][d] * (T)coefF(0, cc) + D[1][cc] * (T)coefF(0, d) +
-                     D[0][d] * (T)coefF(1, cc) + D[0][cc] * (T)coefF(1, d));
+        q = fma(D[1][d], (T)(16.0 * coefF(0, cc)),
+                fma(D[1][cc], (T)(16.0 * coefF(0, d)),
+                    fma(D[0][d], (T)(16.0 * coefF(1, cc)), D[0][cc] * (T)(16.0 * coefF(1, d)))));
       }
     }
-    if (MASS) q += md * tab<T>((6 * NL + LROW) * NL + m);
+    if (MASS) q = fma(md, tab<T>((6 * NL + LROW) * NL + m), q);
     v[m] = q;
   }
 }
 
 // ROWS rows per workgroup, LPR lanes per row, the block's CSR entries staged
-// in LDS.  The lanes of a row take its incidences in turn (lane sub takes
-// sub, sub + LPR, ...) and add their rows of the element matrices in LPR
-// ordered phases, so every entry sums its contributions in incidence order,
-// independent of ROWS and LPR.  Software pipeline over a lane's incidences:
-// while incidence j is contracted, the element record of j+1 and the
-// incidence record of j+3 are in flight.
-template <int NL, int LROW, typename T, bool MASS, bool EXACT, int ROWS, int LPR, typename Rec>
+// in LDS (plus a 256-slot dummy region that takes the contributions of
+// padding steps and eliminated columns).  The lanes of a row take its
+// incidences in turn (lane sub takes sub, sub + LPR, ...) and add their rows
+// of the element matrices in LPR ordered phases, so every entry sums its
+// contributions in incidence order, independent of ROWS and LPR.  An
+// incidence reads its record, then only the relabelled edge values its row
+// needs (gathered by address from the element record, no selects).
+// Software pipeline over a lane's incidences: while incidence j is added, the
+// edge values of j+1 and the record of j+3 are in flight.
+template <int NL, int LROW, typename T, bool MASS, bool EXACT, bool ELIM, int ROWS, int LPR,
+          typename Rec>
 __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
     int32_t row_begin, int32_t row_end, int32_t dummy, EdgePack ep,
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ inc_ptr,
@@ -385,6 +347,7 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
   extern __shared__ __align__(16) char lds_raw[];
   T *acc = reinterpret_cast<T *>(lds_raw);
   constexpr int NT = ROWS * LPR;
+  constexpr int GS = geo_stride<T, MASS>();
   const int lane = threadIdx.x;
   const int sub = lane % LPR;
   const int r0 = row_begin + xcd_chunk(blockIdx.x, gridDim.x) * ROWS;
@@ -401,38 +364,51 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
     // Three slots rotate; unrolled by three so every load lands in the slot
     // that consumes it and no loaded register is copied.  The body has no
     // branch: indices past the row's last incidence are clamped (loads stay
-    // in bounds) and their contribution goes to the dummy slot with value 0.
+    // in bounds) and their contribution goes to the dummy region.
     const int last = iend > i - sub ? iend - 1 : 0;  // a row without incidences reads record 0
     auto at = [&](int j) { return j < iend ? j : last; };
     struct Slot {
       RecWords<NL> r;
-      Geo<T, MASS> g;
+      T e[6];
+      T md;
+    };
+    auto load_vals = [&](Slot &A) {
+      const int l = A.r.el & 15;
+      uint32_t w;
+      if (LROW == 0) {
+        const uint32_t a = (l & 1) ? ep.w[1] : ep.w[0], b = (l & 1) ? ep.w[3] : ep.w[2];
+        w = (l & 2) ? b : a;
+      } else {
+        w = pick6(ep.w + 4, (uint32_t)(l - 4));
+      }
+      const T *g = geo + (size_t)(A.r.el >> 4) * GS;
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+        A.e[j] = edge_needed<LROW, EXACT>(j) ? g[(w >> (3 * j)) & 7u] : (T)0;
+      A.md = MASS ? g[6] : (T)0;
     };
     Slot S0, S1, S2;
     auto step = [&](Slot &A, Slot &B, int j) {
-      B.g = load_geo<T, MASS>(geo, B.r.el >> 4);
-      const bool valid = j < iend;
+      load_vals(B);
       T v[NL];
-      row_values<NL, LROW, T, MASS, EXACT>(A.g, A.r.el & 15, ep, v);
-#pragma unroll
-      for (int m = 0; m < NL; ++m) v[m] = valid ? v[m] : (T)0;
+      row_values<NL, LROW, T, MASS, EXACT>(A.e, A.md, v);
+      const bool valid = j < iend;
+      const int bsel = valid ? base : dummy;
       int idx[NL];
 #pragma unroll
       for (int m = 0; m < NL; ++m) {
         const int pm = A.r.pos(m);
-        idx[m] = (pm == 0xFF || !valid) ? dummy : base + pm;
+        idx[m] = bsel + pm;
+        if (ELIM) idx[m] = pm == 0xFF ? dummy : idx[m];
       }
-      // The lanes of a row are in one wavefront, whose LDS operations execute
-      // in issue order; the compiler barrier keeps the phases from being
-      // merged into one update.
+      // One LDS atomic add per entry.  The lanes of a row are in one
+      // wavefront, whose LDS operations execute in issue order; the compiler
+      // barrier keeps the phases in that order.
 #pragma unroll
       for (int ph = 0; ph < LPR; ++ph) {
         if (LPR == 1 || sub == ph) {
-          T a[NL];
 #pragma unroll
-          for (int m = 0; m < NL; ++m) a[m] = acc[idx[m]];
-#pragma unroll
-          for (int m = 0; m < NL; ++m) acc[idx[m]] = a[m] + v[m];
+          for (int m = 0; m < NL; ++m) unsafeAtomicAdd(&acc[idx[m]], v[m]);
         }
         if (LPR > 1) asm volatile("" ::: "memory");
       }
@@ -441,7 +417,7 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
     S0.r = load_rec<NL>(rec, at(i));
     S1.r = load_rec<NL>(rec, at(i + LPR));
     S2.r = load_rec<NL>(rec, at(i + 2 * LPR));
-    S0.g = load_geo<T, MASS>(geo, S0.r.el >> 4);
+    load_vals(S0);
     for (; i < iend; i += 3 * LPR) {
       step(S0, S1, i);
       step(S1, S2, i + LPR);
@@ -464,7 +440,7 @@ template <bool V>
 using BC = std::integral_constant<bool, V>;
 
 template <int NL, typename T, typename Rec>
-static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals) {
+static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool elim) {
   const bool stiff = op->terms & MG_TERM_STIFF, mass = op->terms & MG_TERM_MASS;
   const double ksc = stiff ? op->stiff_const : 0.0, kmc = mass ? op->mass_const : 0.0;
   const double *ksf = (stiff && op->stiff_coeff == MG_COEFF_P0) ? c->field[MG_FIELD_STIFF] : nullptr;
@@ -493,10 +469,16 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals) {
   const Rec *rec = (const Rec *)S.inc_rec;
   for (const RowSegment &sg : S.segs) {
     const int nblk = (sg.row_end - sg.row_begin + sg.rows - 1) / sg.rows;
-    const size_t lds = sizeof(T) * (size_t)(sg.lds_slots + 1);
+    const size_t lds = sizeof(T) * (size_t)(sg.lds_slots + 256);
     auto go = [&](auto lr, auto ms, auto ex, auto rw, auto lp) {
-      k_asm_rows<NL, decltype(lr)::value, T, decltype(ms)::value, decltype(ex)::value,
-                 decltype(rw)::value, decltype(lp)::value, Rec>
+      if (elim)
+        k_asm_rows<NL, decltype(lr)::value, T, decltype(ms)::value, decltype(ex)::value, true,
+                   decltype(rw)::value, decltype(lp)::value, Rec>
+            <<<nblk, decltype(rw)::value * decltype(lp)::value, lds, c->stream>>>(
+                sg.row_begin, sg.row_end, sg.lds_slots, ep, S.rowptr, S.inc_ptr, rec, geo, vals);
+      else
+        k_asm_rows<NL, decltype(lr)::value, T, decltype(ms)::value, decltype(ex)::value, false,
+                   decltype(rw)::value, decltype(lp)::value, Rec>
           <<<nblk, decltype(rw)::value * decltype(lp)::value, lds, c->stream>>>(
               sg.row_begin, sg.row_end, sg.lds_slots, ep, S.rowptr, S.inc_ptr, rec, geo, vals);
     };
@@ -542,9 +524,9 @@ static void assemble_t(Ctx *c, int sys, const mg_operator *op, int prec) {
   upload_tables(c, c->order, stiff ? op->stiff_rule : 0, mass ? op->mass_rule : 0, prec);
   T *vals = prec == MG_PREC_F64 ? (T *)S.vals64 : (T *)S.vals32;
   if (c->nloc == 10)
-    launch_rows<10, T, RecP2>(c, S, op, vals);
+    launch_rows<10, T, RecP2>(c, S, op, vals, sys == MG_SYS_INTERIOR);
   else
-    launch_rows<4, T, RecP1>(c, S, op, vals);
+    launch_rows<4, T, RecP1>(c, S, op, vals, sys == MG_SYS_INTERIOR);
 }
 
 // ---- adjoint element kernel ---------------------------------------------------

@@ -349,7 +349,7 @@ static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvr
       int mx = 0;
       for (int64_t r = a; r < b; r += rows)
         mx = std::max(mx, rp[std::min<int64_t>(r + rows, b)] - rp[r]);
-      if ((mx + 1) * 8 <= 64 * 1024 || rows == 64) {
+      if ((mx + 256) * 8 <= 64 * 1024 || rows == 64) {
         S.segs.push_back(RowSegment{(int32_t)a, (int32_t)b, mx, lrow, rows, lpr});
         return;
       }
@@ -476,8 +476,8 @@ void build_pattern(Ctx *c, int sys) {
     MG_HIP(hipStreamSynchronize(c->stream));
   }
   for (const RowSegment &sg : S.segs)
-    MG_CHECK(sg.lds_slots * 8 <= 64 * 1024 - 64, MG_ERR_SIZE,
-             "pattern: a 64-row block owns more than 8184 CSR entries");
+    MG_CHECK((sg.lds_slots + 256) * 8 <= 64 * 1024, MG_ERR_SIZE,
+             "pattern: a 64-row block owns more than 7936 CSR entries");
   S.vals64 = dalloc<double>(c, S.nnz);
   S.vals32 = dalloc<float>(c, S.nnz);
   S.dinv64 = dalloc<double>(c, nrows);
