@@ -68,6 +68,10 @@ class HipTimer:
         assert self.hip.hipMemset(p, 0, C.c_size_t(nbytes)) == 0
         return p.value
 
+    def free(self, ptr: int):
+        self.hip.hipFree.argtypes = [C.c_void_p]
+        assert self.hip.hipFree(C.c_void_p(ptr)) == 0
+
     def to_host(self, ptr: int, n: int, dtype=np.float64) -> np.ndarray:
         a = np.zeros(n, dtype=dtype)
         assert self.hip.hipMemcpy(a.ctypes.data_as(C.c_void_p), C.c_void_p(ptr),
@@ -97,6 +101,18 @@ def bytes_assembly(T, V, nloc, nnz_full, vb):
 
 def bytes_pcg_iter(n, nnz, vb):
     return nnz * (vb + 4) + 4 * (n + 1) + 13 * vb * n
+
+
+def measured_traffic(n):
+    """HBM bytes of one FP64 assembly at this size from the committed counter
+    summary (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the copy kernel's
+    reading); None when no summary for this size exists."""
+    path = os.path.join(ROOT, "profiles", "assembly_p2_%d_traffic.json" % n)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("bytes_per_assembly")
 
 
 def cpu_baseline(problem, budget_s=20.0):
@@ -171,6 +187,21 @@ def main():
         % (ms_step, wall * 1e3 / args.steps, value))
     b_asm = bytes_assembly(m.nT, m.nV, 10, nnz, 8)
     achieved = b_asm / (ms_step * 1e-3) / 1e9
+
+    # attainable HBM bandwidth: a plain 1 GiB device copy on the same stream
+    cbytes = 1 << 30
+    src, dst = timer.zeros(cbytes), timer.zeros(cbytes)
+    ctx.copy_device(dst, src, cbytes)
+    timer.sync()
+    timer.start()
+    for _ in range(5):
+        ctx.copy_device(dst, src, cbytes)
+    ms_copy = timer.stop_ms() / 5
+    copy_gbs = 2 * cbytes / (ms_copy * 1e-3) / 1e9
+    log("device copy 1 GiB: %.3f ms, %.0f GB/s (read + write)" % (ms_copy, copy_gbs))
+    timer.free(src)
+    timer.free(dst)
+    traffic = measured_traffic(args.n)
 
     # FP32 assembly (inner-solve matrix)
     assemble(prec=L.PREC_F32)
@@ -253,8 +284,11 @@ def main():
                    "n": args.n, "tets": m.nT, "dofs": ndofs, "nnz_full": nnz,
                    "dofs_interior": nint, "nnz_interior": nnz_int},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "assembly (k_geom + k_asm_rows), compulsory bytes %d" % b_asm},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": "one FP64 assembly = k_elem_factors + k_asm_rows (vertex rows) + "
+                               "k_asm_rows (edge rows); algorithmic bytes %d" % b_asm,
+                     "copy_GBps": round(copy_gbs, 1)},
         "assembly_f32_ms": round(ms_asm32, 4),
         "symbolic_s": round(t_sym, 3),
         "cpu_baseline": cpu,

@@ -102,6 +102,11 @@ const char *mg_last_error(const mg_ctx *ctx);
 /* the context's HIP stream (hipStream_t) for callers that enqueue their own work */
 void *mg_ctx_stream(mg_ctx *ctx);
 int mg_ctx_synchronize(mg_ctx *ctx);
+/* Measurement utility, no counterpart in the reference: a plain device copy of
+ * nbytes (a multiple of 16, 16-byte aligned pointers) on the context's stream,
+ * one read and one write of every byte.  Calibrates hardware byte counters and
+ * gives the attainable HBM copy bandwidth next to the kernels' rooflines. */
+int mg_copy_device(mg_ctx *ctx, void *dst, const void *src, int64_t nbytes);
 
 /* ---- mesh ----------------------------------------------------------------
  * replaces meshInit3d::gmMesh_to_stdB (meshInit3d.hpp:487-690) fed by the gmsh

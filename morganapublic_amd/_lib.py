@@ -39,6 +39,7 @@ SOLVE_F32_REFINE = 1
 # every symbol include/mgfe.h declares
 EXPORTS = (
     "mg_ctx_create", "mg_ctx_destroy", "mg_last_error", "mg_ctx_stream", "mg_ctx_synchronize",
+    "mg_copy_device",
     "mg_mesh_set", "mg_field_set", "mg_field_set_device", "mg_dofmap_build", "mg_elem_dofs_get",
     "mg_edges_get", "mg_dirichlet_set", "mg_dirichlet_mask_get", "mg_pattern_build",
     "mg_pattern_get", "mg_slot_map_get", "mg_assemble", "mg_values_get", "mg_values_device",
@@ -102,6 +103,7 @@ def _declare(L):
         "mg_ctx_create": [_I, C.POINTER(_P)],
         "mg_ctx_destroy": [_P],
         "mg_ctx_synchronize": [_P],
+        "mg_copy_device": [_P, _P, _P, _I64],
         "mg_mesh_set": [_P, _I64, _P, _I64, _P, _I64, _P, _P],
         "mg_field_set": [_P, _I, _P],
         "mg_field_set_device": [_P, _I, _P],
@@ -189,6 +191,10 @@ class Context:
 
     def synchronize(self):
         self._chk(self._L.mg_ctx_synchronize(self.h))
+
+    def copy_device(self, dst: int, src: int, nbytes: int):
+        """Plain device copy on the context's stream (bandwidth calibration)."""
+        self._chk(self._L.mg_copy_device(self.h, dst, src, nbytes))
 
     # -- mesh / fields ---------------------------------------------------------
     def mesh_set(self, xyz, tets, btri=None, bgeo=None):

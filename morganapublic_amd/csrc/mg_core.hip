@@ -236,6 +236,14 @@ __global__ void k_btri_check(int64_t nB, int64_t nV, const int32_t *__restrict__
   if (v < 0 || v >= nV) atomicAdd(bad, 1);
 }
 
+// grid-stride 16-byte copy (one read, one write per byte)
+__global__ __launch_bounds__(256) void k_copy16(int64_t n, const int4 *__restrict__ src,
+                                                int4 *__restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 static void reset_derived(Ctx *c) {
   for (int s = 0; s < 2; ++s) {
     free_system(c, c->sys[s]);
@@ -311,6 +319,20 @@ void *mg_ctx_stream(mg_ctx *c) { return c ? (void *)c->stream : nullptr; }
 int mg_ctx_synchronize(mg_ctx *c) {
   API_BEGIN(c)
   MG_HIP(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+
+int mg_copy_device(mg_ctx *c, void *dst, const void *src, int64_t nbytes) {
+  API_BEGIN(c)
+  MG_CHECK(dst && src && nbytes >= 0 && nbytes % 16 == 0 && (uintptr_t)dst % 16 == 0 &&
+               (uintptr_t)src % 16 == 0,
+           MG_ERR_ARG, "copy: 16-byte aligned pointers and a multiple of 16 bytes required");
+  const int64_t n = nbytes / 16;
+  if (n > 0) {
+    const int blocks = (int)std::min<int64_t>(blocks_for(n, 256), 256 * 32);
+    k_copy16<<<blocks, 256, 0, c->stream>>>(n, (const int4 *)src, (int4 *)dst);
+    MG_HIP(hipGetLastError());
+  }
   API_END(c)
 }
 

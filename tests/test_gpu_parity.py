@@ -368,3 +368,14 @@ def test_state_errors(ctx):
         ctx.assemble(L.SYS_FULL, L.TERM_STIFF, stiff_coeff=L.COEFF_P0)  # no field
     with pytest.raises(L.MgfeError):
         ctx.assemble(L.SYS_FULL, L.TERM_STIFF, stiff_rule=7)
+
+
+def test_copy_device(ctx):
+    import torch
+    src = torch.arange(1 << 20, dtype=torch.float64, device="cuda")
+    dst = torch.zeros_like(src)
+    ctx.copy_device(dst.data_ptr(), src.data_ptr(), src.numel() * 8)
+    ctx.synchronize()
+    assert torch.equal(src, dst)
+    with pytest.raises(L.MgfeError):
+        ctx.copy_device(dst.data_ptr(), src.data_ptr(), 12)
