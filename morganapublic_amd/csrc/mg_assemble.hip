@@ -401,14 +401,23 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
         idx[m] = bsel + pm;
         if (ELIM) idx[m] = pm == 0xFF ? dummy : idx[m];
       }
-      // One LDS atomic add per entry.  The lanes of a row are in one
-      // wavefront, whose LDS operations execute in issue order; the compiler
-      // barrier keeps the phases in that order.
+      // FP64: one LDS atomic add (ds_add_f64, no return) per entry; FP32: read,
+      // add, write (ds_add_f32 is several times slower here).  The lanes of a
+      // row are in one wavefront, whose LDS operations execute in issue order;
+      // the compiler barrier keeps the phases in that order.
 #pragma unroll
       for (int ph = 0; ph < LPR; ++ph) {
         if (LPR == 1 || sub == ph) {
+          if constexpr (sizeof(T) == 8) {
 #pragma unroll
-          for (int m = 0; m < NL; ++m) unsafeAtomicAdd(&acc[idx[m]], v[m]);
+            for (int m = 0; m < NL; ++m) unsafeAtomicAdd(&acc[idx[m]], v[m]);
+          } else {
+            T a[NL];
+#pragma unroll
+            for (int m = 0; m < NL; ++m) a[m] = acc[idx[m]];
+#pragma unroll
+            for (int m = 0; m < NL; ++m) acc[idx[m]] = a[m] + v[m];
+          }
         }
         if (LPR > 1) asm volatile("" ::: "memory");
       }
@@ -425,7 +434,9 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
     }
   }
   __syncthreads();
-  for (int s = lane; s < ns; s += NT) vals[s0 + s] = acc[s];
+  // values are written once and not read again by this assembly: streaming
+  // (nontemporal) stores keep them from evicting element records from L2
+  for (int s = lane; s < ns; s += NT) __builtin_nontemporal_store(acc[s], vals + s0 + s);
 }
 
 // rules exact on the term's integrand (affine element, P0 coefficient)
