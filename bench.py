@@ -115,6 +115,90 @@ def measured_traffic(n):
     return d.get("bytes_per_assembly")
 
 
+def bench_config5(ctx, timer, n=128):
+    """128^3 P1, lognormal k: forward + adjoint solve, both FP32 inner + FP64
+    refinement to 1e-10, vs the FP64 plain method; dJ/dk for J = ||u - u_obs||^2."""
+    p = W.config_problem(5, n=n)
+    m = p.mesh
+    ctx.mesh_set(m.xyz, m.tets, m.btri, m.bgeo)
+    ctx.dofmap_build(1)
+    ctx.dirichlet_set(list(p.dirichlet_geo))
+    nint, nnz_int = ctx.pattern_build(L.SYS_INTERIOR)
+    ctx.field_set(L.FIELD_STIFF, p.k)
+    fixed = ctx.dirichlet_mask()
+    db = timer.to_device(np.ascontiguousarray(p.f[fixed == 0]))
+    uobs = (0.01 * (2 * W.stream_uniform(0, W.STREAM_UOBS, fixed.size) - 1))[fixed == 0]
+    du, dl, dg = timer.zeros(8 * nint), timer.zeros(8 * nint), timer.zeros(8 * m.nT)
+    out = {"tets": m.nT, "dofs_interior": nint, "nnz_interior": nnz_int}
+    grads = {}
+    for name, mode in (("f64", L.SOLVE_F64), ("f32_refine", L.SOLVE_F32_REFINE)):
+        kw = dict(stiff_coeff=L.COEFF_P0, stiff_rule=1)
+        timer.sync()
+        timer.start()
+        ctx.assemble(L.SYS_INTERIOR, L.TERM_STIFF, **kw)
+        if mode == L.SOLVE_F32_REFINE:
+            ctx.assemble(L.SYS_INTERIOR, L.TERM_STIFF, prec=L.PREC_F32, **kw)
+        fi = ctx.pcg_device(db, du, mode=mode, tol=1e-10)
+        ms_fwd = timer.stop_ms()
+        drhs = timer.to_device(2.0 * (timer.to_host(du, nint) - uobs))
+        timer.sync()
+        timer.start()
+        ai = ctx.pcg_device(drhs, dl, mode=mode, tol=1e-10)
+        ctx.adjoint_grad_k_device(L.SYS_INTERIOR, du, dl, dg, rule=1)
+        ms_adj = timer.stop_ms()
+        grads[name] = timer.to_host(dg, m.nT)
+        timer.free(drhs)
+        out[name] = {"forward_ms": round(ms_fwd, 2), "gradient_ms": round(ms_adj, 2),
+                     "grad_fwd_ratio": round(ms_adj / ms_fwd, 3),
+                     "forward_iters": fi["iters"], "adjoint_iters": ai["iters"],
+                     "forward_relres": fi["relres"], "adjoint_relres": ai["relres"]}
+        log("cfg5 %s: forward %.1f ms (%d it), gradient %.1f ms (%d it)"
+            % (name, ms_fwd, fi["iters"], ms_adj, ai["iters"]))
+    a, b = grads["f32_refine"], grads["f64"]
+    out["grad_rel_err_f32_refine_vs_f64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    for ptr in (db, du, dl, dg):
+        timer.free(ptr)
+    return out
+
+
+def bench_config4(ctx, timer, n=96, reps=5):
+    """96^3 P1 drift-diffusion: dual-mesh data, SG + Poisson residual with its
+    loss, and the FP64 VJP w.r.t. (eps, mu), device buffers."""
+    p = W.config_drift_diffusion(n)
+    m = p.mesh
+    ctx.mesh_set(m.xyz, m.tets, m.btri, m.bgeo)
+    ctx.dofmap_build(1)
+    ctx.dirichlet_set(list(p.dirichlet_geo))
+    fixed = ctx.dirichlet_mask()
+    p.fix_state(fixed)
+    timer.sync()
+    t0 = time.perf_counter()
+    ctx.dual_fv_build()
+    timer.sync()
+    t_dual = time.perf_counter() - t0
+    dev = [timer.to_device(a) for a in (p.eps, p.rho, p.mu, p.phi, p.dens)]
+    dfix = timer.to_device(fixed.astype(np.uint8))
+    rp, rn, loss = timer.zeros(8 * m.nV), timer.zeros(8 * m.nV), timer.zeros(8)
+    ge, gm = timer.zeros(8 * m.nT), timer.zeros(8 * m.nT)
+    args = dev + [dfix]
+    ctx.dd_residual_device(*args, rp, rn, loss)
+    timer.sync()
+    timer.start()
+    for _ in range(reps):
+        ctx.dd_residual_device(*args, rp, rn, loss)
+    ms_res = timer.stop_ms() / reps
+    timer.start()
+    for _ in range(reps):
+        ctx.dd_vjp_device(*args, rp, rn, ge, gm)
+    ms_vjp = timer.stop_ms() / reps
+    log("cfg4: dual data %.1f ms, residual+loss %.3f ms, residual+VJP %.3f ms"
+        % (t_dual * 1e3, ms_res, ms_vjp))
+    for ptr in args + [rp, rn, loss, ge, gm]:
+        timer.free(ptr)
+    return {"tets": m.nT, "vertices": m.nV, "dual_build_ms": round(t_dual * 1e3, 2),
+            "residual_loss_ms": round(ms_res, 3), "residual_vjp_ms": round(ms_vjp, 3)}
+
+
 def cpu_baseline(problem, budget_s=20.0):
     """FP64 oracle (the reference algorithm restated in C: per-quadrature-point
     geometry, COO insert + sort/merge) on the first elements of the same mesh,
@@ -144,6 +228,8 @@ def main():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 96^3 drift-diffusion and 128^3 gradient extras")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -264,6 +350,9 @@ def main():
             "pcg_f64_roofline_frac": round(it_bytes64 * res["f64"]["iters"] / (res["f64"]["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         }
 
+    if not args.no_solve and not args.no_configs:
+        out_extra["config5_gradient_128"] = bench_config5(ctx, timer)
+        out_extra["config4_drift_diffusion_96"] = bench_config4(ctx, timer)
     cpu = None if args.no_cpu else cpu_baseline(p)
     line = {
         "metric": "M elem/s assembly; PCG time to 1e-10 (FP32+refine vs FP64); grad/fwd ratio",

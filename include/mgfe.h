@@ -208,6 +208,16 @@ int mg_dd_residual(mg_ctx *ctx, const double *eps, const double *rho, const doub
 int mg_dd_vjp(mg_ctx *ctx, const double *eps, const double *rho, const double *mu,
               const double *phi, const double *dens, const uint8_t *fixed, double *g_eps,
               double *g_mu);
+/* The same on device buffers (the context's GPU), queued on the context
+ * stream without host synchronisation.  r_phi, r_n (nV) receive the
+ * residuals; loss, when not null, is a device double receiving the same
+ * value mg_dd_residual returns. */
+int mg_dd_residual_device(mg_ctx *ctx, const double *eps, const double *rho, const double *mu,
+                          const double *phi, const double *dens, const uint8_t *fixed,
+                          double *r_phi, double *r_n, double *loss);
+int mg_dd_vjp_device(mg_ctx *ctx, const double *eps, const double *rho, const double *mu,
+                     const double *phi, const double *dens, const uint8_t *fixed, double *r_phi,
+                     double *r_n, double *g_eps, double *g_mu);
 /* Bernoulli function B(x) = x / (e^x - 1) in FP64 on the device (n values)  */
 int mg_bernoulli(mg_ctx *ctx, int64_t n, const double *x, double *y);
 

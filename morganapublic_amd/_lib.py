@@ -45,7 +45,7 @@ EXPORTS = (
     "mg_pattern_get", "mg_slot_map_get", "mg_assemble", "mg_values_get", "mg_values_device",
     "mg_spmv", "mg_spmv_device", "mg_pcg_solve", "mg_pcg_solve_device", "mg_adjoint_grad_k",
     "mg_adjoint_grad_k_device", "mg_dual_fv_build", "mg_dual_fv_get", "mg_dd_residual",
-    "mg_dd_vjp", "mg_bernoulli",
+    "mg_dd_vjp", "mg_dd_residual_device", "mg_dd_vjp_device", "mg_bernoulli",
 )
 
 
@@ -128,6 +128,8 @@ def _declare(L):
         "mg_dual_fv_get": [_P, _P, _P, _P, _P],
         "mg_dd_residual": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
         "mg_dd_vjp": [_P, _P, _P, _P, _P, _P, _P, _P, _P],
+        "mg_dd_residual_device": [_P] + [_P] * 9,
+        "mg_dd_vjp_device": [_P] + [_P] * 10,
         "mg_bernoulli": [_P, _I64, _P, _P],
     }
     for name, args in sig.items():
@@ -368,6 +370,15 @@ class Context:
         ge, gm = np.zeros(self.nT), np.zeros(self.nT)
         self._chk(self._L.mg_dd_vjp(self.h, *[_ptr(x) for x in a], _ptr(fx), _ptr(ge), _ptr(gm)))
         return ge, gm
+
+    def dd_residual_device(self, eps, rho, mu, phi, dens, fixed, r_phi, r_n, loss=0):
+        """Device-pointer form (ints); queued on the context stream."""
+        self._chk(self._L.mg_dd_residual_device(self.h, *[_P(p) for p in (
+            eps, rho, mu, phi, dens, fixed, r_phi, r_n)], _P(loss) if loss else None))
+
+    def dd_vjp_device(self, eps, rho, mu, phi, dens, fixed, r_phi, r_n, g_eps, g_mu):
+        self._chk(self._L.mg_dd_vjp_device(self.h, *[_P(p) for p in (
+            eps, rho, mu, phi, dens, fixed, r_phi, r_n, g_eps, g_mu)]))
 
     def bernoulli(self, x):
         x = _arr(x, np.float64)

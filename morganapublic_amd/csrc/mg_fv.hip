@@ -333,6 +333,16 @@ __global__ void k_loss(int64_t n, const double *__restrict__ a, const double *__
   if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
+// sum of the block partials in index order on one lane (the same order the
+// host-side loss used), so both entry points return the same bits
+__global__ void k_sum_in_order(int n, const double *__restrict__ partial, double *__restrict__ out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += partial[i];
+    *out = s;
+  }
+}
+
 struct DDBuffers {
   double *eps, *rho, *mu, *phi, *dens, *rphi, *rn, *ge, *gm, *partial;
   uint8_t *fixed;
@@ -351,7 +361,7 @@ static DDBuffers dd_upload(Ctx *c, const double *eps, const double *rho, const d
   B.dens = dalloc<double>(c, nV);
   B.rphi = dalloc<double>(c, nV);
   B.rn = dalloc<double>(c, nV);
-  B.partial = dalloc<double>(c, 1024);
+  B.partial = dalloc<double>(c, 1025);
   B.fixed = dalloc<uint8_t>(c, nV);
   MG_HIP(hipMemcpyAsync(B.eps, eps, sizeof(double) * nT, hipMemcpyHostToDevice, c->stream));
   MG_HIP(hipMemcpyAsync(B.rho, rho, sizeof(double) * nT, hipMemcpyHostToDevice, c->stream));
@@ -438,14 +448,10 @@ int mg_dd_residual(mg_ctx *c, const double *eps, const double *rho, const double
     MG_HIP(hipMemcpyAsync(r_n, B.rn, sizeof(double) * c->nV, hipMemcpyDeviceToHost, c->stream));
   if (loss) {
     k_loss<<<1024, 256, 0, c->stream>>>(c->nV, B.rphi, B.rn, B.partial);
+    k_sum_in_order<<<1, 64, 0, c->stream>>>(1024, B.partial, B.partial + 1024);
     MG_HIP(hipGetLastError());
-    std::vector<double> part(1024);
-    MG_HIP(hipMemcpyAsync(part.data(), B.partial, sizeof(double) * 1024, hipMemcpyDeviceToHost,
+    MG_HIP(hipMemcpyAsync(loss, B.partial + 1024, sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
-    MG_HIP(hipStreamSynchronize(c->stream));
-    double s = 0.0;
-    for (double v : part) s += v;
-    *loss = s;
   }
   MG_HIP(hipStreamSynchronize(c->stream));
   dd_free(c, B);
@@ -468,6 +474,43 @@ int mg_dd_vjp(mg_ctx *c, const double *eps, const double *rho, const double *mu,
   MG_HIP(hipMemcpyAsync(g_mu, B.gm, sizeof(double) * c->nT, hipMemcpyDeviceToHost, c->stream));
   MG_HIP(hipStreamSynchronize(c->stream));
   dd_free(c, B);
+  API_END(c)
+}
+
+int mg_dd_residual_device(mg_ctx *c, const double *eps, const double *rho, const double *mu,
+                          const double *phi, const double *dens, const uint8_t *fixed,
+                          double *r_phi, double *r_n, double *loss) {
+  API_BEGIN(c)
+  MG_CHECK(eps && rho && mu && phi && dens && fixed && r_phi && r_n, MG_ERR_ARG,
+           "dd: null argument");
+  MG_CHECK(c->dual_ok, MG_ERR_STATE, "dd: dual data not built");
+  DDBuffers B{const_cast<double *>(eps), const_cast<double *>(rho), const_cast<double *>(mu),
+              const_cast<double *>(phi), const_cast<double *>(dens), r_phi, r_n, nullptr, nullptr,
+              nullptr, const_cast<uint8_t *>(fixed)};
+  dd_residual_launch(c, B);
+  if (loss) {
+    B.partial = (double *)scratch(c, sizeof(double) * 1024);
+    k_loss<<<1024, 256, 0, c->stream>>>(c->nV, B.rphi, B.rn, B.partial);
+    k_sum_in_order<<<1, 64, 0, c->stream>>>(1024, B.partial, loss);
+    MG_HIP(hipGetLastError());
+  }
+  API_END(c)
+}
+
+int mg_dd_vjp_device(mg_ctx *c, const double *eps, const double *rho, const double *mu,
+                     const double *phi, const double *dens, const uint8_t *fixed, double *r_phi,
+                     double *r_n, double *g_eps, double *g_mu) {
+  API_BEGIN(c)
+  MG_CHECK(eps && rho && mu && phi && dens && fixed && r_phi && r_n && g_eps && g_mu,
+           MG_ERR_ARG, "dd: null argument");
+  MG_CHECK(c->dual_ok, MG_ERR_STATE, "dd: dual data not built");
+  DDBuffers B{const_cast<double *>(eps), const_cast<double *>(rho), const_cast<double *>(mu),
+              const_cast<double *>(phi), const_cast<double *>(dens), r_phi, r_n, g_eps, g_mu,
+              nullptr, const_cast<uint8_t *>(fixed)};
+  dd_residual_launch(c, B);
+  k_dd_vjp<<<blocks_for(c->nT, 128), 128, 0, c->stream>>>(c->nT, c->tets, c->xyz, c->sigma_el,
+                                                          B.phi, B.dens, B.rphi, B.rn, B.ge, B.gm);
+  MG_HIP(hipGetLastError());
   API_END(c)
 }
 

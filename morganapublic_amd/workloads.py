@@ -256,3 +256,35 @@ def config_problem(cfg: int, n: int | None = None, seed: int = 0) -> Problem:
         f = 2.0 * stream_uniform(seed, STREAM_F, mesh.nV) - 1.0
         return Problem("p1_diffusion_cube%d" % n, 1, mesh, k, False, f, (1, 2, 3, 4, 5, 6))
     raise ValueError("no linear-solve workload for config %r" % cfg)
+
+
+@dataclasses.dataclass
+class DriftDiffusionProblem:
+    """Configuration 4: P1 Poisson coefficients, Scharfetter-Gummel mobility and
+    a state (phi, n) on a jittered Kuhn cube.  phi is prescribed (= x) on the
+    faces x = 0 and x = 1 (geoIds 1, 2); the caller applies it with the
+    library's Dirichlet mask (fix_state)."""
+    name: str
+    mesh: TetMesh
+    eps: np.ndarray   # per element, U[1, 5]
+    rho: np.ndarray   # per element, U[-1, 1]
+    mu: np.ndarray    # per element, exp(N(0, 0.5^2))
+    phi: np.ndarray   # per vertex, x + 0.1 U[-1, 1]
+    dens: np.ndarray  # per vertex, exp(N(0, 1))
+    dirichlet_geo: tuple = (1, 2)
+
+    def fix_state(self, fixed: np.ndarray) -> None:
+        """phi = x on the vertices the Dirichlet mask marks."""
+        sel = np.asarray(fixed) != 0
+        self.phi[sel] = self.mesh.xyz[sel, 0]
+
+
+def config_drift_diffusion(n: int = 96, seed: int = 0) -> DriftDiffusionProblem:
+    """Workload of configuration 4 at size n (96 by default)."""
+    mesh = kuhn_cube(n, jitter=0.2, seed=seed)
+    eps = 1.0 + 4.0 * stream_uniform(seed, STREAM_K, mesh.nT)
+    rho = 2.0 * stream_uniform(seed, STREAM_RHO, mesh.nT) - 1.0
+    mu = np.exp(0.5 * stream_normal(seed, STREAM_MU, mesh.nT))
+    dens = np.exp(stream_normal(seed, STREAM_DENSITY, mesh.nV))
+    phi = mesh.xyz[:, 0] + 0.1 * (2.0 * stream_uniform(seed, STREAM_PHI, mesh.nV) - 1.0)
+    return DriftDiffusionProblem("p1_drift_diffusion_cube%d" % n, mesh, eps, rho, mu, phi, dens)

@@ -1,0 +1,119 @@
+"""Benchmark configurations 3, 4 and 5 at their full sizes, through the C ABI.
+
+The oracle cannot assemble 1.5-12.6 M elements in a test, so these checks use
+size-independent properties: FP32 values against the FP64 values (which the
+parity tests pin to the oracle at smaller sizes), explicit FP64 residuals of
+the solves, the mixed-precision gradient against the FP64 plain-method
+gradient, and central finite differences of the drift-diffusion loss.
+"""
+import numpy as np
+import pytest
+
+from morganapublic_amd import _lib as L
+from morganapublic_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _row_scaled_max(rp, a, b):
+    rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+    scale = np.zeros(rp.size - 1)
+    np.maximum.at(scale, rows, np.abs(b))
+    return float((np.abs(a - b) / scale[rows]).max())
+
+
+def _residual(rp, ci, va, x, b):
+    import scipy.sparse as sp
+    n = rp.size - 1
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    return float(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+
+
+def test_config3_fp32_assembly_and_mixed_solve(ctx):
+    """64^3 P2: FP32 element contraction and values vs FP64; FP32-inner PCG
+    with FP64 refinement reaches the same 1e-10 FP64 residual."""
+    p = W.config_problem(3)
+    m = p.mesh
+    ctx.mesh_set(m.xyz, m.tets, m.btri, m.bgeo)
+    ctx.dofmap_build(2)
+    ctx.dirichlet_set(list(p.dirichlet_geo))
+    ctx.pattern_build(L.SYS_INTERIOR)
+    ctx.field_set(L.FIELD_STIFF, p.k)
+    kw = dict(stiff_coeff=L.COEFF_P0, stiff_rule=11)
+    ctx.assemble(L.SYS_INTERIOR, L.TERM_STIFF, **kw)
+    ctx.assemble(L.SYS_INTERIOR, L.TERM_STIFF, prec=L.PREC_F32, **kw)
+    rp, ci = ctx.pattern(L.SYS_INTERIOR)
+    v64 = ctx.values(L.SYS_INTERIOR)
+    v32 = ctx.values(L.SYS_INTERIOR, prec=L.PREC_F32)
+    assert _row_scaled_max(rp, v32.astype(np.float64), v64) <= 2e-6
+    fixed = ctx.dirichlet_mask()
+    b = p.f[fixed == 0]
+    x64, info64 = ctx.pcg(b, mode=L.SOLVE_F64, tol=1e-10)
+    x32, info32 = ctx.pcg(b, mode=L.SOLVE_F32_REFINE, tol=1e-10)
+    print("cfg3 f64 iters %d, mixed iters %d / %d refinements"
+          % (info64["iters"], info32["iters"], info32["outer"]), flush=True)
+    for x, info in ((x64, info64), (x32, info32)):
+        assert info["status"] == L.MG_OK
+        assert info["relres"] <= 1e-10
+        assert _residual(rp, ci, v64, x, b) <= 1.5e-10
+    assert np.linalg.norm(x32 - x64) <= 1e-8 * np.linalg.norm(x64)
+
+
+def test_config4_vjp_central_differences(ctx):
+    """96^3 P1 drift-diffusion: the FP64 VJP of loss = sum R^2 w.r.t. eps and
+    mu against central differences (h = 1e-6 relative) on 64 random elements."""
+    p = W.config_drift_diffusion(96)
+    m = p.mesh
+    ctx.mesh_set(m.xyz, m.tets, m.btri, m.bgeo)
+    ctx.dofmap_build(1)
+    ctx.dirichlet_set(list(p.dirichlet_geo))
+    fixed = ctx.dirichlet_mask()
+    p.fix_state(fixed)
+    ctx.dual_fv_build()
+    args = (p.rho,)
+    ge, gm = ctx.dd_vjp(p.eps, p.rho, p.mu, p.phi, p.dens, fixed)
+    rng = np.random.default_rng(4)
+    worst = 0.0
+    for e in rng.choice(m.nT, 64, replace=False):
+        for which, g in (("eps", ge), ("mu", gm)):
+            base = p.eps if which == "eps" else p.mu
+            h = 1e-6 * base[e]
+            vals = []
+            for s in (1.0, -1.0):
+                a = base.copy()
+                a[e] += s * h
+                eps, mu = (a, p.mu) if which == "eps" else (p.eps, a)
+                vals.append(ctx.dd_residual(eps, *args, mu, p.phi, p.dens, fixed)[0])
+            fd = (vals[0] - vals[1]) / (2 * h)
+            scale = max(abs(g[e]), 1e-3 * np.abs(g).max())
+            worst = max(worst, abs(fd - g[e]) / scale)
+    print("cfg4 worst FD mismatch (scaled) %.3e" % worst, flush=True)
+    assert worst <= 1e-4
+
+
+def test_config5_mixed_gradient_matches_fp64(ctx):
+    """128^3 P1: forward + adjoint solve with FP32 inner / FP64 refinement to
+    1e-10 gives the FP64 plain-method gradient dJ/dk, J = ||u - u_obs||^2."""
+    torch = pytest.importorskip("torch")
+    from morganapublic_amd.autograd import diffusion_solve
+    p = W.config_problem(5)
+    m = p.mesh
+    ctx.mesh_set(m.xyz, m.tets, m.btri, m.bgeo)
+    ctx.dofmap_build(1)
+    ctx.dirichlet_set(list(p.dirichlet_geo))
+    ctx.pattern_build(L.SYS_INTERIOR)
+    fixed = ctx.dirichlet_mask()
+    f = torch.tensor(p.f[fixed == 0], dtype=torch.float64, device="cuda")
+    uobs = torch.tensor(0.01 * (2 * W.stream_uniform(0, W.STREAM_UOBS, fixed.size) - 1)[fixed == 0],
+                        dtype=torch.float64, device="cuda")
+    k = torch.tensor(p.k, dtype=torch.float64, device="cuda")
+    grads = {}
+    for mode in (L.SOLVE_F64, L.SOLVE_F32_REFINE):
+        kk = k.clone().requires_grad_(True)
+        u = diffusion_solve(kk, f, ctx, mode=mode, rule=1, tol=1e-10)
+        ((u - uobs) ** 2).sum().backward()
+        grads[mode] = kk.grad.cpu().numpy()
+    a, b = grads[L.SOLVE_F32_REFINE], grads[L.SOLVE_F64]
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print("cfg5 mixed vs FP64 gradient rel %.3e" % rel, flush=True)
+    assert rel < 1e-7

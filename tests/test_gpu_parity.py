@@ -379,3 +379,26 @@ def test_copy_device(ctx):
     assert torch.equal(src, dst)
     with pytest.raises(L.MgfeError):
         ctx.copy_device(dst.data_ptr(), src.data_ptr(), 12)
+
+
+def test_dd_device_entry_points_match_host(ctx):
+    import torch
+    m, eps, rho, mu, phi, dens, fixed = _dd()
+    setup(ctx, m, 1, None)
+    ctx.dual_fv_build()
+    loss, rp, rn = ctx.dd_residual(eps, rho, mu, phi, dens, fixed)
+    ge, gm = ctx.dd_vjp(eps, rho, mu, phi, dens, fixed)
+    d = {k: torch.tensor(v, device="cuda") for k, v in
+         dict(eps=eps, rho=rho, mu=mu, phi=phi, dens=dens).items()}
+    dfix = torch.tensor(fixed.astype(np.uint8), device="cuda")
+    out = {k: torch.zeros(n, dtype=torch.float64, device="cuda") for k, n in
+           dict(rp=m.nV, rn=m.nV, ge=m.nT, gm=m.nT, loss=1).items()}
+    args = [d[k].data_ptr() for k in ("eps", "rho", "mu", "phi", "dens")] + [dfix.data_ptr()]
+    ctx.dd_residual_device(*args, out["rp"].data_ptr(), out["rn"].data_ptr(), out["loss"].data_ptr())
+    ctx.synchronize()
+    assert out["loss"].item() == loss
+    assert np.array_equal(out["rp"].cpu().numpy(), rp) and np.array_equal(out["rn"].cpu().numpy(), rn)
+    ctx.dd_vjp_device(*args, out["rp"].data_ptr(), out["rn"].data_ptr(), out["ge"].data_ptr(),
+                      out["gm"].data_ptr())
+    ctx.synchronize()
+    assert np.array_equal(out["ge"].cpu().numpy(), ge) and np.array_equal(out["gm"].cpu().numpy(), gm)
