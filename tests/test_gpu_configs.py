@@ -61,7 +61,13 @@ def test_config3_fp32_assembly_and_mixed_solve(ctx):
 
 def test_config4_vjp_central_differences(ctx):
     """96^3 P1 drift-diffusion: the FP64 VJP of loss = sum R^2 w.r.t. eps and
-    mu against central differences (h = 1e-6 relative) on 64 random elements."""
+    mu against central differences (h = 1e-6) on 64 random elements.
+
+    At this size one element moves the loss (~10) by ~1e-11, below the
+    rounding of the full sum, so the difference of the two losses is formed
+    vertex by vertex, (L+ - L-) = sum_v (R+_v - R-_v)(R+_v + R-_v): the
+    residuals are deterministic, so every vertex outside the element
+    contributes exactly zero and no cancellation is left."""
     p = W.config_drift_diffusion(96)
     m = p.mesh
     ctx.mesh_set(m.xyz, m.tets, m.btri, m.bgeo)
@@ -70,25 +76,27 @@ def test_config4_vjp_central_differences(ctx):
     fixed = ctx.dirichlet_mask()
     p.fix_state(fixed)
     ctx.dual_fv_build()
-    args = (p.rho,)
     ge, gm = ctx.dd_vjp(p.eps, p.rho, p.mu, p.phi, p.dens, fixed)
     rng = np.random.default_rng(4)
+    h = 1e-6
     worst = 0.0
     for e in rng.choice(m.nT, 64, replace=False):
         for which, g in (("eps", ge), ("mu", gm)):
             base = p.eps if which == "eps" else p.mu
-            h = 1e-6 * base[e]
-            vals = []
+            res = []
             for s in (1.0, -1.0):
                 a = base.copy()
                 a[e] += s * h
                 eps, mu = (a, p.mu) if which == "eps" else (p.eps, a)
-                vals.append(ctx.dd_residual(eps, *args, mu, p.phi, p.dens, fixed)[0])
-            fd = (vals[0] - vals[1]) / (2 * h)
+                _, rp, rn = ctx.dd_residual(eps, p.rho, mu, p.phi, p.dens, fixed)
+                res.append((rp, rn))
+            (pp, pn), (mp, mn) = res
+            dl = np.sum((pp - mp) * (pp + mp)) + np.sum((pn - mn) * (pn + mn))
+            fd = dl / (2 * h)
             scale = max(abs(g[e]), 1e-3 * np.abs(g).max())
             worst = max(worst, abs(fd - g[e]) / scale)
     print("cfg4 worst FD mismatch (scaled) %.3e" % worst, flush=True)
-    assert worst <= 1e-4
+    assert worst <= 1e-5
 
 
 def test_config5_mixed_gradient_matches_fp64(ctx):
