@@ -64,6 +64,10 @@ extern "C" {
 
 #define MG_SOLVE_F64 0        /* plain FP64 Jacobi-PCG                        */
 #define MG_SOLVE_F32_REFINE 1 /* FP32 inner Jacobi-PCG, FP64 refinement      */
+/* FP32 Jacobi-PCG whose residual is replaced by the FP64 residual b - A x
+ * (x accumulated in FP64) each time the recursive residual has dropped by
+ * inner_tol; the search direction is kept across replacements. */
+#define MG_SOLVE_F32_RELIABLE 2
 
 typedef struct mg_ctx mg_ctx;
 
@@ -78,9 +82,10 @@ typedef struct {
 } mg_operator;
 
 typedef struct {
-  int mode;          /* MG_SOLVE_F64 or MG_SOLVE_F32_REFINE                   */
+  int mode;          /* MG_SOLVE_F64, MG_SOLVE_F32_REFINE or MG_SOLVE_F32_RELIABLE */
   double tol;        /* ||b - A x||_2 / ||b||_2 target (FP64 residual)          */
-  double inner_tol;  /* relative tolerance of each FP32 inner solve            */
+  double inner_tol;  /* F32_REFINE: relative tolerance of each FP32 inner solve;
+                        F32_RELIABLE: residual drop between FP64 replacements */
   int max_iters;     /* bound on the total number of (inner) CG iterations     */
   int max_outer;     /* bound on refinement steps (F32_REFINE)                 */
   int check_every;   /* iterations queued between host convergence checks     */

@@ -35,6 +35,7 @@ PREC_F64 = 0
 PREC_F32 = 1
 SOLVE_F64 = 0
 SOLVE_F32_REFINE = 1
+SOLVE_F32_RELIABLE = 2
 
 # every symbol include/mgfe.h declares
 EXPORTS = (
@@ -307,10 +308,12 @@ class Context:
 
     @staticmethod
     def _params(mode, tol, inner_tol, max_iters, max_outer, check_every, use_x0):
-        return SolverParams(mode, tol, inner_tol, max_iters, max_outer, check_every,
-                            1 if use_x0 else 0)
+        # inner_tol None: the library's default for the mode (1e-5 restarted
+        # refinement, 1e-2 residual drop between FP64 replacements)
+        return SolverParams(mode, tol, 0.0 if inner_tol is None else inner_tol, max_iters,
+                            max_outer, check_every, 1 if use_x0 else 0)
 
-    def pcg(self, b, mode=SOLVE_F64, tol=1e-10, inner_tol=1e-5, max_iters=100000, max_outer=30,
+    def pcg(self, b, mode=SOLVE_F64, tol=1e-10, inner_tol=None, max_iters=100000, max_outer=30,
             check_every=32, x0=None, raise_on_fail=True):
         b = _arr(b, np.float64)
         x = np.zeros_like(b) if x0 is None else _arr(x0, np.float64).copy()
@@ -321,7 +324,7 @@ class Context:
             self._chk(rc)
         return x, info.as_dict()
 
-    def pcg_device(self, b_ptr: int, x_ptr: int, mode=SOLVE_F64, tol=1e-10, inner_tol=1e-5,
+    def pcg_device(self, b_ptr: int, x_ptr: int, mode=SOLVE_F64, tol=1e-10, inner_tol=None,
                    max_iters=100000, max_outer=30, check_every=32, use_x0=False,
                    raise_on_fail=True):
         prm = self._params(mode, tol, inner_tol, max_iters, max_outer, check_every, use_x0)

@@ -23,7 +23,7 @@ class DiffusionSolve(torch.autograd.Function):
         torch.cuda.synchronize()
         lib_ctx.field_set_device(L.FIELD_STIFF, k.data_ptr())
         lib_ctx.assemble(L.SYS_INTERIOR, L.TERM_STIFF, stiff_coeff=L.COEFF_P0, stiff_rule=rule)
-        if mode == L.SOLVE_F32_REFINE:
+        if mode != L.SOLVE_F64:
             lib_ctx.assemble(L.SYS_INTERIOR, L.TERM_STIFF, stiff_coeff=L.COEFF_P0,
                              stiff_rule=rule, prec=L.PREC_F32)
         u = torch.zeros_like(f)

@@ -433,6 +433,63 @@ __global__ void k_axpy_f32(int64_t n, double a, const float *__restrict__ d, dou
   if (i < n) x[i] += a * (double)d[i];
 }
 
+// Residual replacement of the FP32 recursion: partials of r.D^{-1}r and r.r
+// of the replaced residual r.
+__global__ __launch_bounds__(kThreads) void k_replace_partials(int n, const float *__restrict__ r,
+                                                               const float *__restrict__ dinv,
+                                                               double *__restrict__ rz_out,
+                                                               double *__restrict__ rr_out) {
+  __shared__ double sh[kThreads / 64];
+  double a1 = 0.0, a2 = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const float ri = r[i];
+    a1 += (double)(ri * (dinv[i] * ri));
+    a2 += (double)(ri * ri);
+  }
+  const double b1 = block_sum_all(a1, sh);
+  const double b2 = block_sum_all(a2, sh);
+  if (threadIdx.x == 0) {
+    rz_out[blockIdx.x] = b1;
+    rr_out[blockIdx.x] = b2;
+  }
+}
+
+// The direction update the recursion skipped when it stopped, made with the
+// replaced residual r (new units, r = r64 / s_new) and p still in the old
+// units (s_old): in new units p' = pscale p with pscale = s_old / s_new and
+// r.z_old' = pscale^2 r.z_old, so p = D^{-1} r + beta p' with
+// beta p' = rz_repl / (rz_old pscale) p.
+__global__ __launch_bounds__(kThreads) void k_replace_pupdate(int n, const float *__restrict__ r,
+                                                              float *__restrict__ p,
+                                                              const float *__restrict__ dinv,
+                                                              const double *__restrict__ rz_repl,
+                                                              const double *__restrict__ rz_old,
+                                                              int np, double pscale) {
+  __shared__ double sh[kThreads / 64];
+  const double a = total(rz_repl, np, sh);
+  const double o = total(rz_old, np, sh);
+  const float beta = (float)(a / (o * pscale));
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+    p[i] = dinv[i] * r[i] + beta * p[i];
+}
+
+// one workgroup: the recursion restarts its stopping test from the replaced
+// residual (tol2 = delta^2 r.r), keeping the iteration count
+__global__ void k_replace_state(const double *__restrict__ rz_part,
+                                const double *__restrict__ rr_part, int np, double delta2,
+                                PcgState *__restrict__ st) {
+  __shared__ double sh[kThreads / 64];
+  const double rz = total(rz_part, np, sh);
+  const double rr = total(rr_part, np, sh);
+  if (threadIdx.x == 0) {
+    st->rz = rz;
+    st->rr = rr;
+    st->tol2 = delta2 * rr;
+    st->status = MG_OK;
+    st->done = 0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 static int lanes_per_row(const System &S) {
   const double avg = (double)S.nnz / (double)S.nrows;
@@ -609,6 +666,104 @@ static PcgState pcg_dispatch(Ctx *c, System &S, const T *vals, const T *dinv, co
 }
 
 static double true_relres(Ctx *c, System &S, const double *b, const double *x, Workspace &w,
+                          double bnorm);
+
+// FP32 Jacobi-PCG whose residual is replaced by the FP64 one whenever the
+// recursive residual has dropped by delta since the last replacement: the
+// iterate is accumulated in FP64 (x += s x32), r = b - A x is recomputed with
+// the FP64 matrix, and the recursion continues with the same search
+// direction (rescaled), so no Krylov information is thrown away as it is by
+// restarted refinement.  A replacement that does not reduce the true residual
+// restarts the direction (p = D^{-1} r).  Returns the number of FP32
+// iterations; *relres / *status / *outer as in pcg_solve.
+template <int LPR>
+static int pcg_reliable(Ctx *c, System &S, const double *b, double *x, double bnorm, double tol,
+                        double delta, int maxit, int check_every, Workspace &w, double *relres,
+                        int *status, int *outer) {
+  const int n = (int)S.nrows;
+  const int gr = grid_rows(n, LPR), gv = grid_vec(n), gb = grid_blocks(S);
+  Partials &P = w.part;
+  const int chunk = check_every > 0 ? check_every : 32;
+  *outer = 0;
+  *status = MG_ERR_NOCONV;
+  double rel = true_relres(c, S, b, x, w, bnorm);  // w.rr64 = b - A x
+  *relres = rel;
+  if (rel <= tol) {
+    *status = MG_OK;
+    return 0;
+  }
+  double scale = rel * bnorm;  // r32 = r64 / scale, x += scale * x32
+  auto start_direction = [&](int it) {
+    // r32 = r64 / scale, p = D^{-1} r32; r.z partials where iteration `it` reads them
+    k_to_f32<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, w.rr64, 1.0 / scale, w.b32);
+    double *rz_prev = (it & 1) ? P.rz0 : P.rz1;
+    k_start<float, LPR><<<gr, kThreads, 0, c->stream>>>(n, S.rowptr, S.colidx, S.vals32, nullptr,
+                                                        w.b32, w.r32, w.p32, S.dinv32, rz_prev,
+                                                        P.tmp);
+    k_start_finalize<<<1, kThreads, 0, c->stream>>>(rz_prev, P.tmp, gr, w.st);
+    if (gr != gv) k_start_partials_to<<<1, kThreads, 0, c->stream>>>(rz_prev, gr, gv);
+    k_replace_state<<<1, kThreads, 0, c->stream>>>(rz_prev, P.tmp, gr, delta * delta, w.st);
+    MG_HIP(hipGetLastError());
+  };
+  init_state(c, w.st, 0.0, maxit);
+  MG_HIP(hipMemsetAsync(w.x32, 0, sizeof(float) * n, c->stream));
+  start_direction(0);
+  int it = 0;
+  while (true) {
+    for (int k = 0; k < chunk; ++k, ++it) {
+      double *rz_prev = (it & 1) ? P.rz0 : P.rz1;
+      double *rz_new = (it & 1) ? P.rz1 : P.rz0;
+      k_spmv_pq_stream<float><<<gb, kThreads, 0, c->stream>>>(
+          S.n_sblk, S.sblk, S.rowptr, S.colidx, S.vals32, w.p32, w.q32, P.pq, w.st);
+      k_update<float><<<gv, kThreads, 0, c->stream>>>(n, w.x32, w.r32, w.p32, w.q32, S.dinv32,
+                                                      P.pq, gb, rz_prev, gv, rz_new, P.rr, w.st);
+      k_pupdate<float><<<gv, kThreads, 0, c->stream>>>(n, w.r32, w.p32, S.dinv32, rz_new, rz_prev,
+                                                       P.rr, gv, it, w.st);
+    }
+    MG_HIP(hipGetLastError());
+    const PcgState h = read_state(c, w.st);
+    if (!h.done) continue;
+    it = h.iters;  // iterations actually performed (the rest of the chunk was skipped)
+    // fold the FP32 increment into x and replace the residual
+    k_axpy_f32<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, scale, w.x32, x);
+    MG_HIP(hipMemsetAsync(w.x32, 0, sizeof(float) * n, c->stream));
+    MG_HIP(hipGetLastError());
+    const double prev = rel;
+    rel = true_relres(c, S, b, x, w, bnorm);
+    *relres = rel;
+    ++*outer;
+    if (rel <= tol) {
+      *status = MG_OK;
+      return it;
+    }
+    if (h.status == MG_ERR_BREAKDOWN && !(rel < prev)) {
+      *status = MG_ERR_BREAKDOWN;
+      return it;
+    }
+    if (it >= maxit) return it;
+    const double ns = rel * bnorm;
+    if (h.status == MG_OK && rel < 0.5 * prev) {
+      // keep the direction.  The stopped iteration (it - 1) read r.z from
+      // rz_old and wrote the recursive r.z where iteration `it` reads it;
+      // that slot now takes the replaced residual's partials.
+      k_to_f32<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, w.rr64, 1.0 / ns, w.r32);
+      double *rz_next = (it & 1) ? P.rz0 : P.rz1;
+      double *rz_old = (it & 1) ? P.rz1 : P.rz0;
+      k_replace_partials<<<gv, kThreads, 0, c->stream>>>(n, w.r32, S.dinv32, rz_next, P.rr);
+      k_replace_pupdate<<<gv, kThreads, 0, c->stream>>>(n, w.r32, w.p32, S.dinv32, rz_next,
+                                                        rz_old, gv, scale / ns);
+      k_replace_state<<<1, kThreads, 0, c->stream>>>(rz_next, P.rr, gv, delta * delta, w.st);
+      MG_HIP(hipGetLastError());
+      scale = ns;
+    } else {
+      // stagnation or breakdown of the recursion: restart the direction
+      scale = ns;
+      start_direction(it);
+    }
+  }
+}
+
+static double true_relres(Ctx *c, System &S, const double *b, const double *x, Workspace &w,
                           double bnorm) {
   const int n = (int)S.nrows;
   (void)n;
@@ -666,6 +821,21 @@ static void pcg_solve(Ctx *c, const mg_solver_params *prm, const double *b, doub
       }
       if (iters >= maxit) break;
     }
+  } else if (prm->mode == MG_SOLVE_F32_RELIABLE) {
+    if (!S.has32) {
+      k_vals_to_f32<<<blocks_for(S.nnz, 256), 256, 0, c->stream>>>(S.nnz, S.vals64, S.vals32);
+      MG_HIP(hipGetLastError());
+      S.has32 = true;
+      S.dinv32_ok = false;
+    }
+    ensure_dinv(c, MG_SYS_INTERIOR, MG_PREC_F32);
+    const double delta = prm->inner_tol > 0 ? prm->inner_tol : 1e-2;
+    if (lanes_per_row(S) == 8)
+      iters = pcg_reliable<8>(c, S, b, x, bnorm, tol, delta, maxit, prm->check_every, w, &relres,
+                              &status, &outer);
+    else
+      iters = pcg_reliable<4>(c, S, b, x, bnorm, tol, delta, maxit, prm->check_every, w, &relres,
+                              &status, &outer);
   } else {
     if (!S.has32) {
       k_vals_to_f32<<<blocks_for(S.nnz, 256), 256, 0, c->stream>>>(S.nnz, S.vals64, S.vals32);
@@ -786,8 +956,9 @@ int mg_pcg_solve_device(mg_ctx *c, const mg_solver_params *p, const double *b, d
                         mg_solve_info *info) {
   API_BEGIN(c)
   MG_CHECK(p && b && x, MG_ERR_ARG, "solve: null argument");
-  MG_CHECK(p->mode == MG_SOLVE_F64 || p->mode == MG_SOLVE_F32_REFINE, MG_ERR_ARG,
-           "solve: bad mode");
+  MG_CHECK(p->mode == MG_SOLVE_F64 || p->mode == MG_SOLVE_F32_REFINE ||
+               p->mode == MG_SOLVE_F32_RELIABLE,
+           MG_ERR_ARG, "solve: bad mode");
   pcg_solve(c, p, b, x, info);
   if (info && info->status != MG_OK) {
     c->err = info->status == MG_ERR_NOCONV ? "solve: iteration limit reached"
@@ -801,8 +972,9 @@ int mg_pcg_solve(mg_ctx *c, const mg_solver_params *p, const double *b, double *
                  mg_solve_info *info) {
   API_BEGIN(c)
   MG_CHECK(p && b && x, MG_ERR_ARG, "solve: null argument");
-  MG_CHECK(p->mode == MG_SOLVE_F64 || p->mode == MG_SOLVE_F32_REFINE, MG_ERR_ARG,
-           "solve: bad mode");
+  MG_CHECK(p->mode == MG_SOLVE_F64 || p->mode == MG_SOLVE_F32_REFINE ||
+               p->mode == MG_SOLVE_F32_RELIABLE,
+           MG_ERR_ARG, "solve: bad mode");
   System &S = c->sys[MG_SYS_INTERIOR];
   MG_CHECK(S.built, MG_ERR_STATE, "solve: interior pattern not built");
   const int64_t n = S.nrows;

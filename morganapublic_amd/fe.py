@@ -235,9 +235,14 @@ class BelosCG:
             raise ValueError("Block Size must be 1")
         self.maxit = int(params.get("Maximum Iterations", 100000))
         self.tol = float(params.get("Convergence Tolerance", 1e-10))
-        self.mode = L.SOLVE_F32_REFINE if str(params.get("Precision", "double")).lower() in (
-            "mixed", "float+refine") else L.SOLVE_F64
-        self.inner_tol = float(params.get("Inner Tolerance", 1e-5))
+        prec = str(params.get("Precision", "double")).lower()
+        # "float+refine": FP32 inner solves restarted by FP64 refinement;
+        # "mixed" / "float+reliable": one FP32 recursion with FP64 residual
+        # replacement (keeps the search direction, fewer iterations)
+        self.mode = {"float+refine": L.SOLVE_F32_REFINE, "mixed": L.SOLVE_F32_RELIABLE,
+                     "float+reliable": L.SOLVE_F32_RELIABLE}.get(prec, L.SOLVE_F64)
+        it = params.get("Inner Tolerance")
+        self.inner_tol = None if it is None else float(it)
         self.verbosity = int(params.get("Verbosity", 0))
         self.A = self.x = self.b = None
         self.prec = None

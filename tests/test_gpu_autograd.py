@@ -54,14 +54,15 @@ def test_mixed_gradient_matches_fp64_gradient(ctx):
     from morganapublic_amd.autograd import diffusion_solve
     p, f, uobs, k = _problem(ctx, 1, 12)
     grads = {}
-    for mode in (L.SOLVE_F64, L.SOLVE_F32_REFINE):
+    for mode in (L.SOLVE_F64, L.SOLVE_F32_REFINE, L.SOLVE_F32_RELIABLE):
         kk = k.clone().requires_grad_(True)
         u = diffusion_solve(kk, f, ctx, mode=mode, rule=1, tol=1e-10)
         ((u - uobs) ** 2).sum().backward()
         grads[mode] = kk.grad.cpu().numpy()
-    a, b = grads[L.SOLVE_F32_REFINE], grads[L.SOLVE_F64]
-    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
-    assert rel < 1e-7, rel
+    b = grads[L.SOLVE_F64]
+    for mode in (L.SOLVE_F32_REFINE, L.SOLVE_F32_RELIABLE):
+        rel = np.linalg.norm(grads[mode] - b) / np.linalg.norm(b)
+        assert rel < 1e-7, (mode, rel)
 
 
 def test_grad_wrt_rhs_is_adjoint(ctx):

@@ -49,14 +49,16 @@ def test_config3_fp32_assembly_and_mixed_solve(ctx):
     fixed = ctx.dirichlet_mask()
     b = p.f[fixed == 0]
     x64, info64 = ctx.pcg(b, mode=L.SOLVE_F64, tol=1e-10)
-    x32, info32 = ctx.pcg(b, mode=L.SOLVE_F32_REFINE, tol=1e-10)
-    print("cfg3 f64 iters %d, mixed iters %d / %d refinements"
-          % (info64["iters"], info32["iters"], info32["outer"]), flush=True)
-    for x, info in ((x64, info64), (x32, info32)):
-        assert info["status"] == L.MG_OK
-        assert info["relres"] <= 1e-10
-        assert _residual(rp, ci, v64, x, b) <= 1.5e-10
-    assert np.linalg.norm(x32 - x64) <= 1e-8 * np.linalg.norm(x64)
+    assert info64["status"] == L.MG_OK and info64["relres"] <= 1e-10
+    assert _residual(rp, ci, v64, x64, b) <= 1.5e-10
+    for mode in (L.SOLVE_F32_REFINE, L.SOLVE_F32_RELIABLE):
+        x32, info32 = ctx.pcg(b, mode=mode, tol=1e-10)
+        print("cfg3 mode %d: f64 iters %d, mixed iters %d / %d FP64 residuals"
+              % (mode, info64["iters"], info32["iters"], info32["outer"]), flush=True)
+        assert info32["status"] == L.MG_OK
+        assert info32["relres"] <= 1e-10
+        assert _residual(rp, ci, v64, x32, b) <= 1.5e-10
+        assert np.linalg.norm(x32 - x64) <= 1e-8 * np.linalg.norm(x64)
 
 
 def test_config4_vjp_central_differences(ctx):
@@ -116,12 +118,13 @@ def test_config5_mixed_gradient_matches_fp64(ctx):
                         dtype=torch.float64, device="cuda")
     k = torch.tensor(p.k, dtype=torch.float64, device="cuda")
     grads = {}
-    for mode in (L.SOLVE_F64, L.SOLVE_F32_REFINE):
+    for mode in (L.SOLVE_F64, L.SOLVE_F32_REFINE, L.SOLVE_F32_RELIABLE):
         kk = k.clone().requires_grad_(True)
         u = diffusion_solve(kk, f, ctx, mode=mode, rule=1, tol=1e-10)
         ((u - uobs) ** 2).sum().backward()
         grads[mode] = kk.grad.cpu().numpy()
-    a, b = grads[L.SOLVE_F32_REFINE], grads[L.SOLVE_F64]
-    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
-    print("cfg5 mixed vs FP64 gradient rel %.3e" % rel, flush=True)
-    assert rel < 1e-7
+    b = grads[L.SOLVE_F64]
+    for mode in (L.SOLVE_F32_REFINE, L.SOLVE_F32_RELIABLE):
+        rel = np.linalg.norm(grads[mode] - b) / np.linalg.norm(b)
+        print("cfg5 mode %d vs FP64 gradient rel %.3e" % (mode, rel), flush=True)
+        assert rel < 1e-7
