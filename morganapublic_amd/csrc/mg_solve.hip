@@ -1,10 +1,15 @@
-// CSR SpMV and Jacobi-preconditioned CG (FP64 plain, or FP32 inner solves
-// inside an FP64 iterative refinement).
+// CSR SpMV and Jacobi-preconditioned CG: FP64 plain; FP32 inner solves
+// restarted by FP64 iterative refinement; or one FP32 recursion with FP64
+// residual replacement (pcg_reliable).
 //
-// One CG iteration is three kernels, all grid-stride over a fixed grid:
-//   k_spmv_pq : q = A p and the per-workgroup partials of p.q
-//   k_update  : alpha, x += alpha p, r -= alpha q, partials of r.D^{-1}r, r.r
-//   k_pupdate : convergence test, beta, p = D^{-1} r + beta p
+// SpMV works on row blocks (<= kSpmvRows rows, <= kSpmvNnz entries, fixed at
+// pattern time): the block's (column, value) pairs are streamed with
+// coalesced loads, the products staged in LDS, and each row summed by 4 lanes
+// in a fixed order.  One CG iteration is three kernels, grid-stride over a
+// fixed grid:
+//   k_spmv_pq_stream : q = A p and the per-workgroup partials of p.q
+//   k_update         : alpha, x += alpha p, r -= alpha q, partials of r.D^{-1}r, r.r
+//   k_pupdate        : convergence test, beta, p = D^{-1} r + beta p
 // Every workgroup of a kernel first sums the partials the previous kernel
 // wrote, in the same fixed order, so all workgroups hold bit-identical
 // scalars without communicating inside a kernel (a device-scope fence per
@@ -131,44 +136,6 @@ __device__ __forceinline__ T stream_block(int b, const int32_t *__restrict__ sbl
   *row_out = (row < r1) ? row : -1;
   return sum;
 }
-
-template <typename T, int LPR>
-__global__ __launch_bounds__(kThreads) void k_spmv(int n, const int32_t *__restrict__ rowptr,
-                                                   const int32_t *__restrict__ colidx,
-                                                   const T *__restrict__ vals,
-                                                   const T *__restrict__ x, T *__restrict__ y) {
-  const int lane = threadIdx.x % LPR;
-  const int stride = gridDim.x * (kThreads / LPR);
-  for (int row = blockIdx.x * (kThreads / LPR) + threadIdx.x / LPR; row < n; row += stride) {
-    const T s = row_dot<T, LPR>(row, lane, rowptr, colidx, vals, x);
-    if (lane == 0) y[row] = s;
-  }
-}
-
-// q = A p; partial p.q
-template <typename T, int LPR>
-__global__ __launch_bounds__(kThreads) void k_spmv_pq(int n, const int32_t *__restrict__ rowptr,
-                                                      const int32_t *__restrict__ colidx,
-                                                      const T *__restrict__ vals,
-                                                      const T *__restrict__ p, T *__restrict__ q,
-                                                      double *__restrict__ pq_part,
-                                                      const PcgState *__restrict__ st) {
-  __shared__ double sh[kThreads / 64];
-  if (st->done) return;
-  const int lane = threadIdx.x % LPR;
-  const int stride = gridDim.x * (kThreads / LPR);
-  double acc = 0.0;
-  for (int row = blockIdx.x * (kThreads / LPR) + threadIdx.x / LPR; row < n; row += stride) {
-    const T s = row_dot<T, LPR>(row, lane, rowptr, colidx, vals, p);
-    if (lane == 0) {
-      q[row] = s;
-      acc += (double)(s * p[row]);
-    }
-  }
-  const double b = block_sum_all(acc, sh);
-  if (threadIdx.x == 0) pq_part[blockIdx.x] = b;
-}
-
 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_spmv_stream(int nb, const int32_t *__restrict__ sblk,
@@ -363,31 +330,6 @@ __global__ void k_start_finalize(const double *__restrict__ rz_part,
     st->status = MG_OK;
     st->done = (rr <= st->tol2) ? 1 : 0;
   }
-}
-
-// FP64 residual r = b - A x; partial r.r
-template <int LPR>
-__global__ __launch_bounds__(kThreads) void k_resid64(int n, const int32_t *__restrict__ rowptr,
-                                                      const int32_t *__restrict__ colidx,
-                                                      const double *__restrict__ vals,
-                                                      const double *__restrict__ x,
-                                                      const double *__restrict__ b,
-                                                      double *__restrict__ r,
-                                                      double *__restrict__ part) {
-  __shared__ double sh[kThreads / 64];
-  const int lane = threadIdx.x % LPR;
-  const int stride = gridDim.x * (kThreads / LPR);
-  double rr = 0.0;
-  for (int row = blockIdx.x * (kThreads / LPR) + threadIdx.x / LPR; row < n; row += stride) {
-    const double ax = row_dot<double, LPR>(row, lane, rowptr, colidx, vals, x);
-    if (lane == 0) {
-      const double ri = b[row] - ax;
-      r[row] = ri;
-      rr += ri * ri;
-    }
-  }
-  const double s = block_sum_all(rr, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 __global__ void k_dot_self(int n, const double *__restrict__ v, double *__restrict__ part) {
