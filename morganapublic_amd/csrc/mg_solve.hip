@@ -90,51 +90,109 @@ __device__ __forceinline__ T row_dot(int row, int lane, const int32_t *__restric
   return s;
 }
 
-// CSR-stream product of one row block: the block's entries are streamed by
-// all 256 threads (coalesced) into LDS as vals * x[col], then 4 lanes per row
-// sum their row in a fixed order.  Returns the row value in lane sub == 0 of
-// each 4-lane group (row index in *row_out, -1 when the group has no row).
+// CSR-stream product of row blocks: a block's entries are streamed by all
+// 256 threads (coalesced) into LDS as vals * x[col], then 4 lanes per row sum
+// their row in a fixed order.  The (column, value) loads of the workgroup's
+// next block are issued before the current block is consumed (two register
+// buffers), so the stream never waits on its own latency.
 template <typename T>
-__device__ __forceinline__ T stream_block(int b, const int32_t *__restrict__ sblk,
-                                          const int32_t *__restrict__ rowptr,
-                                          const int32_t *__restrict__ colidx,
-                                          const T *__restrict__ vals, const T *__restrict__ x,
-                                          T *prod, int *row_out) {
-  const int r0 = sblk[b], r1 = sblk[b + 1];
-  const int s0 = rowptr[r0], ns = rowptr[r1] - s0;
-  constexpr int PER = kSpmvNnz / kThreads;
-  if (ns > 0) {
-    // branch-free: every lane issues its PER (column, value) loads, then its PER
-    // gathers; lanes past the block's end re-read entry s0 and store into the
-    // spare LDS slot kSpmvNnz
-    int col[PER];
-    T v[PER];
+struct StreamBlk {
+  static constexpr int PER = kSpmvNnz / kThreads;
+  int r0, r1, s0, ns;
+  int ra, re;  // this lane's row: its entries [ra, re) relative to s0
+  int col[PER];
+  T v[PER];
+  T g[PER];  // x at col
+  T xr;      // the row operand of the epilogue at this lane's row
+};
+
+// branch-free: every lane issues its PER (column, value) loads; lanes past
+// the block's end re-read entry s0 (and later store into the spare LDS slot)
+template <typename T>
+__device__ __forceinline__ void load_blk(StreamBlk<T> &B, int b, const int32_t *__restrict__ sblk,
+                                         const int32_t *__restrict__ rowptr,
+                                         const int32_t *__restrict__ colidx,
+                                         const T *__restrict__ vals) {
+  B.r0 = sblk[b];
+  B.r1 = sblk[b + 1];
+  B.s0 = rowptr[B.r0];
+  B.ns = rowptr[B.r1] - B.s0;
+  const int row = min(B.r0 + (int)(threadIdx.x >> 2), B.r1);  // == r1: empty range
+  B.ra = rowptr[row] - B.s0;
+  B.re = rowptr[min(row + 1, B.r1)] - B.s0;
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int k = threadIdx.x + u * kThreads;
-      const int kc = s0 + (k < ns ? k : 0);
-      col[u] = colidx[kc];
-      v[u] = vals[kc];
-    }
+  for (int u = 0; u < StreamBlk<T>::PER; ++u) {
+    const int k = threadIdx.x + u * kThreads;
+    const int kc = B.s0 + (k < B.ns ? k : 0);
+    B.col[u] = colidx[kc];
+    B.v[u] = vals[kc];
+  }
+}
+
+// Gathers x at the block's columns (issued, not waited for).
+template <typename T>
+__device__ __forceinline__ void gather_blk(StreamBlk<T> &B, const T *__restrict__ x,
+                                           const T *__restrict__ rowvec) {
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
+  for (int u = 0; u < StreamBlk<T>::PER; ++u) B.g[u] = x[B.col[u]];
+  B.xr = rowvec[min(B.r0 + (int)(threadIdx.x >> 2), B.r1 - 1)];
+}
+
+// Stages the products in LDS and sums each row; returns the row value in lane
+// sub == 0 of each 4-lane group (row index in *row_out, -1 when the group has
+// no row).
+template <typename T>
+__device__ __forceinline__ T finish_blk(const StreamBlk<T> &B, T *prod, int *row_out) {
+  if (B.ns > 0) {
+#pragma unroll
+    for (int u = 0; u < StreamBlk<T>::PER; ++u) {
       const int k = threadIdx.x + u * kThreads;
-      prod[k < ns ? k : kSpmvNnz] = v[u] * x[col[u]];
+      prod[k < B.ns ? k : kSpmvNnz] = B.v[u] * B.g[u];
     }
   }
   __syncthreads();
-  const int g = threadIdx.x >> 2, sub = threadIdx.x & 3;
-  const int row = r0 + g;
+  const int row = B.r0 + (int)(threadIdx.x >> 2), sub = threadIdx.x & 3;
   T sum = (T)0;
-  if (row < r1) {
-    const int a = rowptr[row] - s0, e = rowptr[row + 1] - s0;
-    for (int k = a + sub; k < e; k += 4) sum += prod[k];
-  }
+  for (int k = B.ra + sub; k < B.re; k += 4) sum += prod[k];
   sum += __shfl_xor(sum, 1, 4);
   sum += __shfl_xor(sum, 2, 4);
   __syncthreads();
-  *row_out = (row < r1) ? row : -1;
+  *row_out = (row < B.r1) ? row : -1;
   return sum;
+}
+
+// Grid-stride loop over the row blocks, software-pipelined over three
+// blocks: while block b is summed, the gathers of block b + G and the
+// (column, value) loads of block b + 2G are in flight.  Each step waits only
+// for loads issued one step earlier (the vector-memory counter completes in
+// issue order), and every buffer is written by loads only, never copied.
+// epi(row, value, rowvec[row]) runs in lane sub == 0 of each row's group.
+template <typename T, typename Epi>
+__device__ __forceinline__ void stream_blocks(int nb, const int32_t *__restrict__ sblk,
+                                              const int32_t *__restrict__ rowptr,
+                                              const int32_t *__restrict__ colidx,
+                                              const T *__restrict__ vals,
+                                              const T *__restrict__ x,
+                                              const T *__restrict__ rowvec, T *prod, Epi epi) {
+  const int G = gridDim.x;
+  int b = blockIdx.x;
+  if (b >= nb) return;
+  StreamBlk<T> K0, K1, K2;
+  load_blk(K0, b, sblk, rowptr, colidx, vals);
+  load_blk(K1, min(b + G, nb - 1), sblk, rowptr, colidx, vals);
+  gather_blk(K0, x, rowvec);
+  // X: gathered, summed now; Y: loaded, gathered now; Z: loaded now
+  auto step = [&](StreamBlk<T> &X, StreamBlk<T> &Y, StreamBlk<T> &Z) {
+    gather_blk(Y, x, rowvec);
+    load_blk(Z, min(b + 2 * G, nb - 1), sblk, rowptr, colidx, vals);
+    int row;
+    const T s = finish_blk(X, prod, &row);
+    if ((threadIdx.x & 3) == 0 && row >= 0) epi(row, s, X.xr);
+    b += G;
+    return b < nb;
+  };
+  while (step(K0, K1, K2) && step(K1, K2, K0) && step(K2, K0, K1)) {
+  }
 }
 
 template <typename T>
@@ -145,11 +203,8 @@ __global__ __launch_bounds__(kThreads) void k_spmv_stream(int nb, const int32_t 
                                                           const T *__restrict__ x,
                                                           T *__restrict__ y) {
   __shared__ T prod[kSpmvNnz + 1];
-  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
-    int row;
-    const T s = stream_block<T>(b, sblk, rowptr, colidx, vals, x, prod, &row);
-    if ((threadIdx.x & 3) == 0 && row >= 0) y[row] = s;
-  }
+  stream_blocks<T>(nb, sblk, rowptr, colidx, vals, x, x, prod,
+                   [&](int row, T s, T) { y[row] = s; });
 }
 
 // q = A p; partial p.q (row blocks)
@@ -162,14 +217,10 @@ __global__ __launch_bounds__(kThreads) void k_spmv_pq_stream(
   __shared__ double sh[kThreads / 64];
   if (st->done) return;
   double acc = 0.0;
-  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
-    int row;
-    const T s = stream_block<T>(b, sblk, rowptr, colidx, vals, p, prod, &row);
-    if ((threadIdx.x & 3) == 0 && row >= 0) {
-      q[row] = s;
-      acc += (double)(s * p[row]);
-    }
-  }
+  stream_blocks<T>(nb, sblk, rowptr, colidx, vals, p, p, prod, [&](int row, T s, T pr) {
+    q[row] = s;
+    acc += (double)(s * pr);
+  });
   const double bsum = block_sum_all(acc, sh);
   if (threadIdx.x == 0) pq_part[blockIdx.x] = bsum;
 }
@@ -183,15 +234,12 @@ __global__ __launch_bounds__(kThreads) void k_resid64_stream(
   __shared__ double prod[kSpmvNnz + 1];
   __shared__ double sh[kThreads / 64];
   double rr = 0.0;
-  for (int blk = blockIdx.x; blk < nb; blk += gridDim.x) {
-    int row;
-    const double ax = stream_block<double>(blk, sblk, rowptr, colidx, vals, x, prod, &row);
-    if ((threadIdx.x & 3) == 0 && row >= 0) {
-      const double ri = b[row] - ax;
-      r[row] = ri;
-      rr += ri * ri;
-    }
-  }
+  stream_blocks<double>(nb, sblk, rowptr, colidx, vals, x, b, prod,
+                        [&](int row, double ax, double br) {
+    const double ri = br - ax;
+    r[row] = ri;
+    rr += ri * ri;
+  });
   const double s = block_sum_all(rr, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }

@@ -9,6 +9,16 @@ for p in (ROOT, os.path.join(ROOT, "oracle")):
         sys.path.insert(0, p)
 
 
+# torch (when installed) is imported before libmgfe.so is loaded, so that the
+# process holds one HIP runtime whichever tests are selected: torch's ROCm
+# build brings its own libamdhip64, which libmgfe.so then shares; loading it
+# after libmgfe.so has started the system runtime leaves torch without devices.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
 
