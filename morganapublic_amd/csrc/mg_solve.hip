@@ -23,7 +23,10 @@
 namespace mg {
 
 static constexpr int kThreads = 256;
-static constexpr int kMaxBlocks = 2048;
+static constexpr int kMaxBlocks = 2048;  // SpMV grid cap = size of every partial array
+// vector-kernel grid cap: every workgroup of k_update / k_pupdate re-totals the
+// previous kernel's partials, so fewer, fuller workgroups read less of them
+static constexpr int kMaxVecBlocks = 1024;
 
 // Host-visible solver state; written by block 0 of a kernel or by a one-block
 // finalize kernel, read by later kernels (kernel boundaries order the writes).
@@ -488,12 +491,12 @@ static int lanes_per_row(const System &S) {
 
 static int grid_rows(int64_t n, int lpr) {
   int64_t b = (n * lpr + kThreads - 1) / kThreads;
-  return (int)(b < kMaxBlocks ? (b < 1 ? 1 : b) : kMaxBlocks);
+  return (int)(b < kMaxVecBlocks ? (b < 1 ? 1 : b) : kMaxVecBlocks);
 }
 
 static int grid_vec(int64_t n) {
   int64_t b = (n + kThreads - 1) / kThreads;
-  return (int)(b < kMaxBlocks ? (b < 1 ? 1 : b) : kMaxBlocks);
+  return (int)(b < kMaxVecBlocks ? (b < 1 ? 1 : b) : kMaxVecBlocks);
 }
 
 void ensure_dinv(Ctx *c, int sys, int prec) {
