@@ -170,6 +170,28 @@ int mg_values_get(mg_ctx *ctx, int sys, int prec, void *vals);
 /* device pointer of the assembled values (double* or float*)               */
 int mg_values_device(mg_ctx *ctx, int sys, int prec, void **vals_device);
 
+/* ---- load vector and Dirichlet lifting -------------------------------------
+ * replaces traitsVectorBuilder::buildEpetraVector (traitsVectorBuilder.hpp:217-274)
+ * over fnEL3dA (f v) integrated by integratorFEL3d_STD
+ * (integratorFEL3d_STD.hpp:99-127): F_j = sum_e int_e f phi_j with `rule`
+ * points, on the rows of `sys` (full or interior numbering).  f: MG_LOAD_CONST
+ * (nf = 1), MG_LOAD_P0 (nf = nT, one value per element) or MG_LOAD_NODAL
+ * (nf = ndofs, a field of the same element type on the full numbering).     */
+#define MG_LOAD_CONST 0
+#define MG_LOAD_P0 1
+#define MG_LOAD_NODAL 2
+int mg_assemble_load(mg_ctx *ctx, int sys, int kind, const double *f, int64_t nf, int rule,
+                     double *F);
+int mg_assemble_load_device(mg_ctx *ctx, int sys, int kind, const double *f, int rule,
+                            double *F);
+/* Non-homogeneous Dirichlet data g (full numbering, read at the fixed dofs):
+ * b (interior numbering) -= A_full[free, fixed] g, with the full FP64 matrix
+ * assembled.  The reference imposes such data through its Lagrange
+ * multiplier system (operatorLA3d.hpp:21-90); elimination moves it to the
+ * right-hand side instead. */
+int mg_dirichlet_lift(mg_ctx *ctx, const double *g, double *b);
+int mg_dirichlet_lift_device(mg_ctx *ctx, const double *g, double *b);
+
 /* ---- linear algebra ----------------------------------------------------------
  * Epetra_CrsMatrix::Apply: y = A x on the assembled system.                */
 int mg_spmv(mg_ctx *ctx, int sys, int prec, const double *x, double *y);

@@ -36,6 +36,7 @@ PREC_F32 = 1
 SOLVE_F64 = 0
 SOLVE_F32_REFINE = 1
 SOLVE_F32_RELIABLE = 2
+LOAD_CONST, LOAD_P0, LOAD_NODAL = 0, 1, 2
 
 # every symbol include/mgfe.h declares
 EXPORTS = (
@@ -44,6 +45,7 @@ EXPORTS = (
     "mg_mesh_set", "mg_field_set", "mg_field_set_device", "mg_dofmap_build", "mg_elem_dofs_get",
     "mg_edges_get", "mg_dirichlet_set", "mg_dirichlet_mask_get", "mg_pattern_build",
     "mg_pattern_get", "mg_slot_map_get", "mg_assemble", "mg_values_get", "mg_values_device",
+    "mg_assemble_load", "mg_assemble_load_device", "mg_dirichlet_lift", "mg_dirichlet_lift_device",
     "mg_spmv", "mg_spmv_device", "mg_pcg_solve", "mg_pcg_solve_device", "mg_adjoint_grad_k",
     "mg_adjoint_grad_k_device", "mg_dual_fv_build", "mg_dual_fv_get", "mg_dd_residual",
     "mg_dd_vjp", "mg_dd_residual_device", "mg_dd_vjp_device", "mg_bernoulli",
@@ -104,6 +106,10 @@ def _declare(L):
         "mg_ctx_create": [_I, C.POINTER(_P)],
         "mg_ctx_destroy": [_P],
         "mg_ctx_synchronize": [_P],
+        "mg_assemble_load": [_P, _I, _I, _P, _I64, _I, _P],
+        "mg_assemble_load_device": [_P, _I, _I, _P, _I, _P],
+        "mg_dirichlet_lift": [_P, _P, _P],
+        "mg_dirichlet_lift_device": [_P, _P, _P],
         "mg_copy_device": [_P, _P, _P, _I64],
         "mg_mesh_set": [_P, _I64, _P, _I64, _P, _I64, _P, _P],
         "mg_field_set": [_P, _I, _P],
@@ -295,6 +301,28 @@ class Context:
         p = _P()
         self._chk(self._L.mg_values_device(self.h, sys, prec, C.byref(p)))
         return p.value
+
+    # -- load vector / Dirichlet lifting -------------------------------------------
+    def assemble_load(self, sys: int, kind: int, f, rule=4):
+        """F_j = int f phi_j on the rows of `sys` (f: scalar, per element or per dof)."""
+        f = _arr(np.atleast_1d(np.asarray(f, np.float64)), np.float64)
+        n, _ = self.sizes(sys)
+        F = np.zeros(n)
+        self._chk(self._L.mg_assemble_load(self.h, sys, kind, _ptr(f), f.size, rule, _ptr(F)))
+        return F
+
+    def assemble_load_device(self, sys: int, kind: int, f_ptr: int, F_ptr: int, rule=4):
+        self._chk(self._L.mg_assemble_load_device(self.h, sys, kind, _P(f_ptr), rule, _P(F_ptr)))
+
+    def dirichlet_lift(self, g, b):
+        """b - A[free, fixed] g (g on the full numbering, b on the interior one)."""
+        g = _arr(g, np.float64)
+        out = _arr(b, np.float64).copy()
+        self._chk(self._L.mg_dirichlet_lift(self.h, _ptr(g), _ptr(out)))
+        return out
+
+    def dirichlet_lift_device(self, g_ptr: int, b_ptr: int):
+        self._chk(self._L.mg_dirichlet_lift_device(self.h, _P(g_ptr), _P(b_ptr)))
 
     # -- linear algebra ----------------------------------------------------------
     def spmv(self, sys: int, x, prec=PREC_F64):

@@ -406,6 +406,58 @@ int or_assemble(int order, int ops, int64_t nV, const double *xyz, int64_t nT,
   return 0;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Load vector F_j = sum_e sum_q w_q det(J_e) f(e, Y_q) phi_j(Y_q), restating  */
+/* traitsVectorBuilder::buildEpetraVector (traitsVectorBuilder.hpp:217-274:   */
+/* element loop, SumIntoGlobalValues at the element's dof gids) over          */
+/* integratorFEL3d_STD::integration (integratorFEL3d_STD.hpp:99-127, signed   */
+/* det) and fnEL3dA::eval (fnEL3dA.hpp, f * v with f = coeff->evalL(el, Y)).   */
+/* kind 0: constant f[0]; 1: P0, f[e]; 2: a field of the same element type,   */
+/* f(Y) = sum_m f[dof(e, m)] phi_m(Y) (f on the full dof numbering).          */
+/* F (ndofs, full numbering) is overwritten.  Returns 0, or -1 (bad rule).    */
+/* ------------------------------------------------------------------------- */
+int or_load_vector(int order, int64_t nV, const double *xyz, int64_t nT, const int32_t *tets,
+                   int kind, const double *f, int rule, double *F) {
+  const int n = order == 1 ? 4 : 10;
+  double pts[3 * 15], w[15];
+  const int nq = or_quad_rule(rule, pts, w);
+  if (!nq) return -1;
+  int32_t *ee = NULL;
+  int64_t N = nV;
+  if (order == 2) {
+    ee = (int32_t *)malloc(sizeof(int32_t) * 6 * (size_t)nT);
+    N = nV + or_edges(nV, nT, tets, ee, NULL);
+  }
+  int32_t *dofs = (int32_t *)malloc(sizeof(int32_t) * n * (size_t)nT);
+  or_elem_dofs(order, nV, nT, tets, ee, dofs);
+  memset(F, 0, sizeof(double) * (size_t)N);
+  double P[12], J[9], phi[10], loc[10];
+  for (int64_t e = 0; e < nT; ++e) {
+    for (int v = 0; v < 4; ++v)
+      for (int d = 0; d < 3; ++d) P[3 * v + d] = xyz[3 * (tets[4 * e + v] - 1) + d];
+    memset(loc, 0, sizeof(loc));
+    for (int q = 0; q < nq; ++q) {
+      or_basis(order, &pts[3 * q], phi, NULL, NULL, NULL);
+      double fv;
+      if (kind == 0) {
+        fv = f[0];
+      } else if (kind == 1) {
+        fv = f[e];
+      } else {
+        fv = 0.0;
+        for (int m = 0; m < n; ++m) fv += f[dofs[n * e + m]] * phi[m];
+      }
+      or_tet_jacobian(P, J);
+      const double jac = or_det3(J);
+      for (int j = 0; j < n; ++j) loc[j] += phi[j] * fv * jac * w[q];
+    }
+    for (int j = 0; j < n; ++j) F[dofs[n * e + j]] += loc[j];
+  }
+  free(dofs);
+  free(ee);
+  return 0;
+}
+
 /* Homogeneous Dirichlet by elimination: A_int = A_full[free, free], free dofs */
 /* renumbered in increasing order.  The reference imposes Dirichlet data with */
 /* Lagrange multipliers instead (operatorLA3d.hpp:21-90); the interior block  */

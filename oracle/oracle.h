@@ -45,6 +45,8 @@ int or_assemble(int order, int ops, int64_t nV, const double *xyz, int64_t nT,
                 const int32_t *tets, const double *c_stiff, const double *c_mass,
                 int rule_stiff, int rule_mass, int64_t *nrows, int64_t *nnz,
                 int32_t **rowptr, int32_t **colidx, double **vals);
+int or_load_vector(int order, int64_t nV, const double *xyz, int64_t nT, const int32_t *tets,
+                   int kind, const double *f, int rule, double *F);
 int or_restrict(int64_t nrows, const int32_t *rowptr, const int32_t *colidx,
                 const double *vals, const uint8_t *fixed, int64_t *nint, int64_t *nnz_int,
                 int32_t **rowptr_int, int32_t **colidx_int, double **vals_int);

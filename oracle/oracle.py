@@ -59,6 +59,8 @@ def _declare(L):
                               C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_P), C.POINTER(_P),
                               C.POINTER(_P)]
     L.or_assemble.restype = _I
+    L.or_load_vector.argtypes = [_I, _I64, _P, _I64, _P, _I, _P, _I, _P]
+    L.or_load_vector.restype = _I
     L.or_restrict.argtypes = [_I64, _P, _P, _P, _P, C.POINTER(_I64), C.POINTER(_I64),
                               C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]
     L.or_restrict.restype = _I
@@ -171,6 +173,20 @@ def assemble(order: int, mesh, ops: int, c_stiff=None, c_mass=None, rule_stiff=4
     if rc:
         raise ValueError("or_assemble failed")
     return _take_csr(nrows, nnz, rp, ci, va)
+
+
+LOAD_CONST, LOAD_P0, LOAD_NODAL = 0, 1, 2
+
+
+def load_vector(order: int, mesh, kind: int, f, rule=4):
+    """F_j = int f phi_j on the full dof numbering (buildEpetraVector over fnEL3dA)."""
+    xyz, tets = _c(mesh.xyz, np.float64), _c(mesh.tets, np.int32)
+    ndofs = mesh.nV if order == 1 else mesh.nV + edges(mesh)[1].shape[0]
+    f = _c(np.atleast_1d(np.asarray(f, np.float64)), np.float64)
+    F = np.zeros(ndofs)
+    if lib().or_load_vector(order, mesh.nV, _p(xyz), mesh.nT, _p(tets), kind, _p(f), rule, _p(F)):
+        raise ValueError("bad rule")
+    return F
 
 
 def restrict(rowptr, colidx, vals, fixed):
