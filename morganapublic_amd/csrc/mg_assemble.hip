@@ -633,10 +633,12 @@ int mg_assemble(mg_ctx *c, int sys, const mg_operator *op, int prec) {
     assemble_t<double>(c, sys, op, prec);
     S.has64 = true;
     S.dinv64_ok = false;
+    S.seq64 = ++c->asm_seq;
   } else {
     assemble_t<float>(c, sys, op, prec);
     S.has32 = true;
     S.dinv32_ok = false;
+    S.seq32 = ++c->asm_seq;
   }
   API_END(c)
 }

@@ -56,6 +56,9 @@ struct System {
   double *vals64 = nullptr;
   float *vals32 = nullptr;
   bool has64 = false, has32 = false;
+  // assembly sequence numbers of the two value arrays: the mixed solvers
+  // refresh the FP32 copy from FP64 values assembled after it
+  uint64_t seq64 = 0, seq32 = 0;
   double *dinv64 = nullptr;
   float *dinv32 = nullptr;
   bool dinv64_ok = false, dinv32_ok = false;
@@ -91,6 +94,7 @@ struct Ctx {
   bool dirichlet_ok = false;
 
   System sys[2];
+  uint64_t asm_seq = 0;  // numbers the assemblies of this context
 
   // per-element geometry/coefficient records for the current assembly
   void *geom = nullptr;

@@ -661,6 +661,18 @@ static PcgState pcg_dispatch(Ctx *c, System &S, const T *vals, const T *dinv, co
 static double true_relres(Ctx *c, System &S, const double *b, const double *x, Workspace &w,
                           double bnorm);
 
+// The mixed solvers' FP32 matrix: the FP32 assembly when it is at least as
+// recent as the FP64 one (the two assemblies of one operator), otherwise the
+// FP64 values rounded, so an FP64 re-assembly never leaves a stale FP32 copy.
+static void refresh_f32(Ctx *c, System &S) {
+  if (S.has32 && S.seq32 >= S.seq64) return;
+  k_vals_to_f32<<<blocks_for(S.nnz, 256), 256, 0, c->stream>>>(S.nnz, S.vals64, S.vals32);
+  MG_HIP(hipGetLastError());
+  S.has32 = true;
+  S.seq32 = S.seq64;
+  S.dinv32_ok = false;
+}
+
 // FP32 Jacobi-PCG whose residual is replaced by the FP64 one whenever the
 // recursive residual has dropped by delta since the last replacement: the
 // iterate is accumulated in FP64 (x += s x32), r = b - A x is recomputed with
@@ -815,12 +827,7 @@ static void pcg_solve(Ctx *c, const mg_solver_params *prm, const double *b, doub
       if (iters >= maxit) break;
     }
   } else if (prm->mode == MG_SOLVE_F32_RELIABLE) {
-    if (!S.has32) {
-      k_vals_to_f32<<<blocks_for(S.nnz, 256), 256, 0, c->stream>>>(S.nnz, S.vals64, S.vals32);
-      MG_HIP(hipGetLastError());
-      S.has32 = true;
-      S.dinv32_ok = false;
-    }
+    refresh_f32(c, S);
     ensure_dinv(c, MG_SYS_INTERIOR, MG_PREC_F32);
     const double delta = prm->inner_tol > 0 ? prm->inner_tol : 1e-2;
     if (lanes_per_row(S) == 8)
@@ -830,12 +837,7 @@ static void pcg_solve(Ctx *c, const mg_solver_params *prm, const double *b, doub
       iters = pcg_reliable<4>(c, S, b, x, bnorm, tol, delta, maxit, prm->check_every, w, &relres,
                               &status, &outer);
   } else {
-    if (!S.has32) {
-      k_vals_to_f32<<<blocks_for(S.nnz, 256), 256, 0, c->stream>>>(S.nnz, S.vals64, S.vals32);
-      MG_HIP(hipGetLastError());
-      S.has32 = true;
-      S.dinv32_ok = false;
-    }
+    refresh_f32(c, S);
     ensure_dinv(c, MG_SYS_INTERIOR, MG_PREC_F32);
     const double itol = prm->inner_tol > 0 ? prm->inner_tol : 1e-5;
     const int max_outer = prm->max_outer > 0 ? prm->max_outer : 30;

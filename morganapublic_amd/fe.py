@@ -12,6 +12,8 @@ libmgfe.so through `Context`.
   matrixBuilder<localMatrixEL3d<...>>::buildEpetraCrs
         (matrixBuilder.hpp:22-60,
          traitsMatrixBuilder.hpp:366-441)           MatrixBuilder(...).buildEpetraCrs()
+  fnEL3dA + traitsVectorBuilder::buildEpetraVector
+        (traitsVectorBuilder.hpp:217-274)           VectorBuilder(space, fnEL3dA(f)).buildEpetraVector()
   factoryPreconditioner + preconditionerIfpack
         (factoryPreconditioner.hpp:53-87,
          preconditionerIfpack.cpp:26-63)            factoryPreconditioner.create(params)
@@ -199,6 +201,41 @@ class MatrixBuilder:
         self.assemble()
         rp, ci = self.space.ctx.pattern(self.sys)
         return CrsMatrix(rp, ci, self.space.ctx.values(self.sys, self.prec))
+
+
+class fnEL3dA:
+    """Functional f v (fnEL3dA.hpp): f a constant, a P0 field (one value per
+    element) or a field of the space's element type (one value per dof)."""
+
+    def __init__(self, coeff=1.0):
+        self.coeff = coeff
+
+    def setCoeff(self, coeff):
+        self.coeff = coeff
+
+
+class VectorBuilder:
+    """vectorBuilder over localVectorEL3d: buildEpetraVector
+    (traitsVectorBuilder.hpp:217-274) with `points` quadrature points."""
+
+    def __init__(self, space: FeSpace, fn: fnEL3dA, points: int = 4, interior: bool = False):
+        self.space, self.fn, self.points = space, fn, points
+        self.sys = L.SYS_INTERIOR if interior else L.SYS_FULL
+        if self.sys not in space.ctx._nnz:
+            space.ctx.pattern_build(self.sys)
+
+    def buildEpetraVector(self) -> np.ndarray:
+        f = np.asarray(self.fn.coeff, np.float64)
+        ctx = self.space.ctx
+        if f.ndim == 0 or f.size == 1:
+            kind = L.LOAD_CONST
+        elif f.size == ctx.nT and f.size != ctx.ndofs:
+            kind = L.LOAD_P0
+        elif f.size == ctx.ndofs:
+            kind = L.LOAD_NODAL
+        else:
+            raise ValueError("coefficient must be a scalar, one value per element or per dof")
+        return ctx.assemble_load(self.sys, kind, f, self.points)
 
 
 class JacobiPreconditioner:

@@ -402,3 +402,21 @@ def test_dd_device_entry_points_match_host(ctx):
                       out["gm"].data_ptr())
     ctx.synchronize()
     assert np.array_equal(out["ge"].cpu().numpy(), ge) and np.array_equal(out["gm"].cpu().numpy(), gm)
+
+
+@pytest.mark.parametrize("mode", [L.SOLVE_F32_REFINE, L.SOLVE_F32_RELIABLE])
+def test_mixed_solve_follows_fp64_reassembly(ctx, mode):
+    # FP64 + FP32 assembled with k1, then FP64 alone with k2: the mixed solver
+    # must use k2 (its FP32 copy is refreshed from the newer FP64 values)
+    p, b = _solve_case(ctx, 5, 6)
+    kw = dict(stiff_coeff=L.COEFF_P0, stiff_rule=1)
+    ctx.field_set(L.FIELD_STIFF, p.k)
+    ctx.assemble(L.SYS_INTERIOR, L.TERM_STIFF, **kw)
+    ctx.assemble(L.SYS_INTERIOR, L.TERM_STIFF, prec=L.PREC_F32, **kw)
+    ctx.pcg(b, mode=mode, tol=1e-10)
+    ctx.field_set(L.FIELD_STIFF, 3.0 * p.k + 1.0)
+    ctx.assemble(L.SYS_INTERIOR, L.TERM_STIFF, **kw)
+    x64, _ = ctx.pcg(b, mode=L.SOLVE_F64, tol=1e-12)
+    x, info = ctx.pcg(b, mode=mode, tol=1e-10)
+    assert info["relres"] <= 1e-10
+    assert np.linalg.norm(x - x64) <= 1e-8 * np.linalg.norm(x64)

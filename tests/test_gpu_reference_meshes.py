@@ -89,3 +89,36 @@ def test_solver_factory_mirror():
         fe.factoryLinearSolver.create({"Solver": "belosGmres"})
     with pytest.raises(ValueError):
         fe.factoryPreconditioner.create({"precClass": "precIfpack", "precType": "ILU"})
+
+
+def test_traits_vector_builder_mirror():
+    # fnEL3dA over a P1 space of a reference mesh: constant, P0 and nodal f
+    path = os.path.join(os.path.dirname(__file__), "golden", "meshes", "mignon3dB.msh")
+    with L.Context(0) as ctx:
+        mesh = fe.Mesh3d.from_gmsh(path)
+        space = fe.FeSpace(ctx, mesh, 1)
+        m = mesh.mesh
+        for f, kind in ((2.5, O.LOAD_CONST), (1.0 + np.arange(m.nT), O.LOAD_P0),
+                        (np.sin(m.xyz[:, 0]) + m.xyz[:, 2], O.LOAD_NODAL)):
+            F = fe.VectorBuilder(space, fe.fnEL3dA(f), points=4).buildEpetraVector()
+            ref = O.load_vector(1, m, kind, f, 4)
+            assert np.abs(F - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_mixed_precision_solver_mirror():
+    p = W.config_problem(1, n=10)
+    with L.Context(0) as ctx:
+        space = fe.FeSpace(ctx, fe.Mesh3d(p.mesh), 1)
+        space.setDirichlet(p.dirichlet_geo)
+        A = fe.MatrixBuilder(space, fe.opEL3dF(1.0) + fe.opEL3dG(), stiff_points=1,
+                             mass_points=4, interior=True)
+        A.assemble()
+        b = p.f[space.fixedMask() == 0]
+        for precision in ("mixed", "float+refine"):
+            x = np.zeros_like(b)
+            solver = fe.factoryLinearSolver.create(
+                {"Solver": "belosCG", "Maximum Iterations": 1000, "Convergence Tolerance": 1e-10,
+                 "Precision": precision})
+            solver.setProblem(A, x, b)
+            assert solver.solve()
+            assert solver.info["relres"] <= 1e-10 and solver.info["outer"] >= 1
