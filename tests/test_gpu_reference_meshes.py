@@ -114,11 +114,15 @@ def test_mixed_precision_solver_mirror():
                              mass_points=4, interior=True)
         A.assemble()
         b = p.f[space.fixedMask() == 0]
+        prec = fe.factoryPreconditioner.create({"precClass": "precIfpack"})
+        prec.initialize()
+        prec.compute()
         for precision in ("mixed", "float+refine"):
             x = np.zeros_like(b)
             solver = fe.factoryLinearSolver.create(
                 {"Solver": "belosCG", "Maximum Iterations": 1000, "Convergence Tolerance": 1e-10,
                  "Precision": precision})
             solver.setProblem(A, x, b)
+            solver.setLeftPrec(prec)
             assert solver.solve()
             assert solver.info["relres"] <= 1e-10 and solver.info["outer"] >= 1
