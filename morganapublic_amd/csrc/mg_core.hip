@@ -474,7 +474,9 @@ int mg_pattern_build(mg_ctx *c, int sys, int64_t *n_rows, int64_t *nnz) {
   MG_CHECK(c->dofs, MG_ERR_STATE, "pattern: dof map not built");
   MG_CHECK(sys == MG_SYS_FULL || c->dirichlet_ok, MG_ERR_STATE,
            "pattern: interior system needs mg_dirichlet_set");
-  build_pattern(c, sys);
+  // a pattern stays valid until the mesh, the dof map or the Dirichlet set
+  // change (those calls release it); rebuilding it would drop its values
+  if (!c->sys[sys].built) build_pattern(c, sys);
   if (n_rows) *n_rows = c->sys[sys].nrows;
   if (nnz) *nnz = c->sys[sys].nnz;
   API_END(c)
