@@ -128,3 +128,37 @@ def test_config5_mixed_gradient_matches_fp64(ctx):
         rel = np.linalg.norm(grads[mode] - b) / np.linalg.norm(b)
         print("cfg5 mode %d vs FP64 gradient rel %.3e" % (mode, rel), flush=True)
         assert rel < 1e-7
+
+
+def test_config2_fp64_matrix_closed_forms(ctx):
+    """64^3 P2 full FP64 matrix (the benchmark's step), by properties the
+    oracle cannot reach at this size but mathematics fixes: symmetry, K 1 = 0,
+    and the energy of linear functions, u_a^T K u_b = sum_e k_e vol_e delta_ab
+    for u_a the interpolant of the coordinate x_a (P2 reproduces it exactly,
+    and the 11-point rule integrates the P2 stiffness exactly)."""
+    import scipy.sparse as sp
+    p = W.config_problem(2)
+    m = p.mesh
+    ctx.mesh_set(m.xyz, m.tets, m.btri, m.bgeo)
+    ctx.dofmap_build(2)
+    ctx.pattern_build(L.SYS_FULL)
+    ctx.field_set(L.FIELD_STIFF, p.k)
+    ctx.assemble(L.SYS_FULL, L.TERM_STIFF, stiff_coeff=L.COEFF_P0, stiff_rule=11)
+    rp, ci = ctx.pattern(L.SYS_FULL)
+    va = ctx.values(L.SYS_FULL)
+    n = rp.size - 1
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    scale = np.abs(va).max()
+    assert abs(A - A.T).max() <= 1e-13 * scale
+    assert np.abs(A @ np.ones(n)).max() <= 1e-12 * scale
+    _, en = ctx.edges()
+    P = m.xyz[m.tets - 1]
+    d = P[:, 1:, :] - P[:, :1, :]
+    vol = np.linalg.det(np.transpose(d, (0, 2, 1))) / 6.0
+    ref = float((p.k * vol).sum())
+    U = [np.concatenate([m.xyz[:, a], 0.5 * (m.xyz[en[:, 0], a] + m.xyz[en[:, 1], a])])
+         for a in range(3)]
+    for a in range(3):
+        for b in range(3):
+            e = float(U[a] @ (A @ U[b]))
+            assert abs(e - (ref if a == b else 0.0)) <= 1e-11 * ref, (a, b, e, ref)
