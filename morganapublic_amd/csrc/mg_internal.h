@@ -35,8 +35,20 @@ struct RowSegment {
   int32_t rows, lpr;  // rows per workgroup, lanes per row
 };
 
-constexpr int kSpmvRows = 64;
+constexpr int kSpmvRows = 64;  // rows per SpMV block (128 for short rows)
 constexpr int kSpmvNnz = 2048;
+// 16-bit column form of the SpMV streams: entry = cluster << 13 | offset
+// against kColClusters bases per row block, each cluster 8192 columns wide
+constexpr int kColClusters = 8;
+constexpr int kColSpan = 1 << 13;
+
+// SpMV row block: rows [r0, r1), CSR entries [s0, s0 + ns), and the column
+// cluster bases of the 16-bit form; 64 bytes, read with one scalar load
+struct alignas(64) BlkDesc {
+  int32_t r0, r1, s0, ns;
+  int32_t base[kColClusters];
+  int32_t pad[4];
+};
 
 struct System {
   bool built = false;
@@ -49,10 +61,14 @@ struct System {
   int32_t *inc_ptr = nullptr;  // nrows + 1 (row -> incidences)
   void *inc_rec = nullptr;     // n_inc records (RecP1 or RecP2), sorted by (row, l, e)
   std::vector<RowSegment> segs;
-  // SpMV row blocks: rows [sblk[b], sblk[b+1]) hold <= kSpmvNnz entries and
-  // <= kSpmvRows rows
-  int32_t *sblk = nullptr;
+  // SpMV row blocks: each holds <= kSpmvNnz entries and <= kSpmvRows rows
+  BlkDesc *sdesc = nullptr;
   int32_t n_sblk = 0;
+  int srl = 4;  // lanes per row summing a block: 4 (64-row blocks) or 2 (128-row blocks)
+  // the columns in 16-bit form (c16: every block fits kColClusters clusters,
+  // bases in sdesc), read by the SpMV streams instead of colidx
+  uint16_t *cofs = nullptr;  // nnz
+  bool c16 = false;
   double *vals64 = nullptr;
   float *vals32 = nullptr;
   bool has64 = false, has32 = false;

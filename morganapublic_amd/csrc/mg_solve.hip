@@ -4,8 +4,9 @@
 //
 // SpMV works on row blocks (<= kSpmvRows rows, <= kSpmvNnz entries, fixed at
 // pattern time): the block's (column, value) pairs are streamed with
-// coalesced loads, the products staged in LDS, and each row summed by 4 lanes
-// in a fixed order.  One CG iteration is three kernels, grid-stride over a
+// coalesced loads (columns in a 16-bit clustered form when the pattern
+// allows it), the products staged in LDS, and each row summed by 4 lanes in a
+// fixed order.  One CG iteration is three kernels, grid-stride over a
 // fixed grid:
 //   k_spmv_pq_stream : q = A p and the per-workgroup partials of p.q
 //   k_update         : alpha, x += alpha p, r -= alpha q, partials of r.D^{-1}r, r.r
@@ -17,6 +18,7 @@
 // state every `check_every` iterations; once converged, queued kernels return
 // at entry.  No reduction uses float atomics.
 #include <cmath>
+#include <type_traits>
 
 #include "mg_internal.h"
 
@@ -98,12 +100,27 @@ __device__ __forceinline__ T row_dot(int row, int lane, const int32_t *__restric
 // their row in a fixed order.  The (column, value) loads of the workgroup's
 // next block are issued before the current block is consumed (two register
 // buffers), so the stream never waits on its own latency.
+//
+// Columns come from colidx (int32) or, when every row block fits
+// kColClusters clusters (C16), from the 16-bit form: entry = cluster << 13 |
+// offset, column = (block descriptor).base[cluster] + offset, which halves
+// the index bytes of the stream.  Both forms give the same columns, hence
+// bit-identical products.  A block's rows, entry range and cluster bases come
+// in one 64-byte descriptor (scalar loads, no dependent rowptr loads); the
+// bases go to a small LDS table per pipeline slot, which the gather stage
+// reads one step later (the finish stage's barriers lie in between).
+struct Cols {
+  const int32_t *idx;   // int32 columns
+  const uint16_t *ofs;  // 16-bit form
+};
+
 template <typename T>
 struct StreamBlk {
   static constexpr int PER = kSpmvNnz / kThreads;
   int r0, r1, s0, ns;
   int ra, re;  // this lane's row: its entries [ra, re) relative to s0
-  int col[PER];
+  int col[PER];  // column, or the raw 16-bit entry until gather_blk decodes it
+  int *tab;      // this slot's cluster-base table in LDS (16-bit form)
   T v[PER];
   T g[PER];  // x at col
   T xr;      // the row operand of the epilogue at this lane's row
@@ -111,40 +128,55 @@ struct StreamBlk {
 
 // branch-free: every lane issues its PER (column, value) loads; lanes past
 // the block's end re-read entry s0 (and later store into the spare LDS slot)
-template <typename T>
-__device__ __forceinline__ void load_blk(StreamBlk<T> &B, int b, const int32_t *__restrict__ sblk,
-                                         const int32_t *__restrict__ rowptr,
-                                         const int32_t *__restrict__ colidx,
+template <typename T, bool C16, int RL>
+__device__ __forceinline__ void load_blk(StreamBlk<T> &B, int b, const BlkDesc *__restrict__ desc,
+                                         const int32_t *__restrict__ rowptr, const Cols cs,
                                          const T *__restrict__ vals) {
-  B.r0 = sblk[b];
-  B.r1 = sblk[b + 1];
-  B.s0 = rowptr[B.r0];
-  B.ns = rowptr[B.r1] - B.s0;
-  const int row = min(B.r0 + (int)(threadIdx.x >> 2), B.r1);  // == r1: empty range
+  const BlkDesc &D = desc[__builtin_amdgcn_readfirstlane(b)];  // uniform: scalar loads
+  B.r0 = D.r0;
+  B.r1 = D.r1;
+  B.s0 = D.s0;
+  B.ns = D.ns;
+  if constexpr (C16) {
+    // issued with the loads above (one wait), not after them
+    int base[kColClusters];
+#pragma unroll
+    for (int i = 0; i < kColClusters; ++i) base[i] = D.base[i];
+    __builtin_amdgcn_sched_barrier(0);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int i = 0; i < kColClusters; ++i) B.tab[i] = base[i];
+    }
+  }
+  const int row = min(B.r0 + (int)(threadIdx.x / RL), B.r1);  // == r1: empty range
   B.ra = rowptr[row] - B.s0;
   B.re = rowptr[min(row + 1, B.r1)] - B.s0;
 #pragma unroll
   for (int u = 0; u < StreamBlk<T>::PER; ++u) {
     const int k = threadIdx.x + u * kThreads;
     const int kc = B.s0 + (k < B.ns ? k : 0);
-    B.col[u] = colidx[kc];
+    B.col[u] = C16 ? (int)cs.ofs[kc] : cs.idx[kc];
     B.v[u] = vals[kc];
   }
 }
 
 // Gathers x at the block's columns (issued, not waited for).
-template <typename T>
+template <typename T, bool C16, int RL>
 __device__ __forceinline__ void gather_blk(StreamBlk<T> &B, const T *__restrict__ x,
                                            const T *__restrict__ rowvec) {
 #pragma unroll
-  for (int u = 0; u < StreamBlk<T>::PER; ++u) B.g[u] = x[B.col[u]];
-  B.xr = rowvec[min(B.r0 + (int)(threadIdx.x >> 2), B.r1 - 1)];
+  for (int u = 0; u < StreamBlk<T>::PER; ++u) {
+    const int e = B.col[u];
+    const int cidx = C16 ? B.tab[(uint32_t)e >> 13] + (e & (kColSpan - 1)) : e;
+    B.g[u] = x[cidx];
+  }
+  B.xr = rowvec[min(B.r0 + (int)(threadIdx.x / RL), B.r1 - 1)];
 }
 
-// Stages the products in LDS and sums each row; returns the row value in lane
-// sub == 0 of each 4-lane group (row index in *row_out, -1 when the group has
-// no row).
-template <typename T>
+// Stages the products in LDS and sums each row with RL lanes; returns the row
+// value in lane sub == 0 of each RL-lane group (row index in *row_out, -1 when
+// the group has no row).
+template <typename T, int RL>
 __device__ __forceinline__ T finish_blk(const StreamBlk<T> &B, T *prod, int *row_out) {
   if (B.ns > 0) {
 #pragma unroll
@@ -154,11 +186,11 @@ __device__ __forceinline__ T finish_blk(const StreamBlk<T> &B, T *prod, int *row
     }
   }
   __syncthreads();
-  const int row = B.r0 + (int)(threadIdx.x >> 2), sub = threadIdx.x & 3;
+  const int row = B.r0 + (int)(threadIdx.x / RL), sub = threadIdx.x % RL;
   T sum = (T)0;
-  for (int k = B.ra + sub; k < B.re; k += 4) sum += prod[k];
-  sum += __shfl_xor(sum, 1, 4);
-  sum += __shfl_xor(sum, 2, 4);
+  for (int k = B.ra + sub; k < B.re; k += RL) sum += prod[k];
+#pragma unroll
+  for (int o = 1; o < RL; o <<= 1) sum += __shfl_xor(sum, o, RL);
   __syncthreads();
   *row_out = (row < B.r1) ? row : -1;
   return sum;
@@ -170,27 +202,33 @@ __device__ __forceinline__ T finish_blk(const StreamBlk<T> &B, T *prod, int *row
 // for loads issued one step earlier (the vector-memory counter completes in
 // issue order), and every buffer is written by loads only, never copied.
 // epi(row, value, rowvec[row]) runs in lane sub == 0 of each row's group.
-template <typename T, typename Epi>
-__device__ __forceinline__ void stream_blocks(int nb, const int32_t *__restrict__ sblk,
-                                              const int32_t *__restrict__ rowptr,
-                                              const int32_t *__restrict__ colidx,
+// A block holds kThreads / RL rows (RL = 4: 64 rows; RL = 2: 128 rows, for
+// patterns whose rows are short).
+template <typename T, bool C16, int RL, typename Epi>
+__device__ __forceinline__ void stream_blocks(int nb, const BlkDesc *__restrict__ desc,
+                                              const int32_t *__restrict__ rowptr, const Cols cs,
                                               const T *__restrict__ vals,
                                               const T *__restrict__ x,
-                                              const T *__restrict__ rowvec, T *prod, Epi epi) {
+                                              const T *__restrict__ rowvec, T *prod,
+                                              int (*tab)[kColClusters], Epi epi) {
   const int G = gridDim.x;
   int b = blockIdx.x;
   if (b >= nb) return;
   StreamBlk<T> K0, K1, K2;
-  load_blk(K0, b, sblk, rowptr, colidx, vals);
-  load_blk(K1, min(b + G, nb - 1), sblk, rowptr, colidx, vals);
-  gather_blk(K0, x, rowvec);
+  K0.tab = tab[0];
+  K1.tab = tab[1];
+  K2.tab = tab[2];
+  load_blk<T, C16, RL>(K0, b, desc, rowptr, cs, vals);
+  load_blk<T, C16, RL>(K1, min(b + G, nb - 1), desc, rowptr, cs, vals);
+  if (C16) __syncthreads();  // K0's cluster bases
+  gather_blk<T, C16, RL>(K0, x, rowvec);
   // X: gathered, summed now; Y: loaded, gathered now; Z: loaded now
   auto step = [&](StreamBlk<T> &X, StreamBlk<T> &Y, StreamBlk<T> &Z) {
-    gather_blk(Y, x, rowvec);
-    load_blk(Z, min(b + 2 * G, nb - 1), sblk, rowptr, colidx, vals);
+    gather_blk<T, C16, RL>(Y, x, rowvec);
+    load_blk<T, C16, RL>(Z, min(b + 2 * G, nb - 1), desc, rowptr, cs, vals);
     int row;
-    const T s = finish_blk(X, prod, &row);
-    if ((threadIdx.x & 3) == 0 && row >= 0) epi(row, s, X.xr);
+    const T s = finish_blk<T, RL>(X, prod, &row);
+    if (threadIdx.x % RL == 0 && row >= 0) epi(row, s, X.xr);
     b += G;
     return b < nb;
   };
@@ -198,29 +236,30 @@ __device__ __forceinline__ void stream_blocks(int nb, const int32_t *__restrict_
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_spmv_stream(int nb, const int32_t *__restrict__ sblk,
-                                                          const int32_t *__restrict__ rowptr,
-                                                          const int32_t *__restrict__ colidx,
-                                                          const T *__restrict__ vals,
-                                                          const T *__restrict__ x,
-                                                          T *__restrict__ y) {
+template <typename T, bool C16, int RL>
+__global__ __launch_bounds__(kThreads) void k_spmv_stream(
+    int nb, const BlkDesc *__restrict__ desc, const int32_t *__restrict__ rowptr,
+    const int32_t *__restrict__ colidx, const uint16_t *__restrict__ cofs, const T *__restrict__ vals, const T *__restrict__ x,
+    T *__restrict__ y) {
   __shared__ T prod[kSpmvNnz + 1];
-  stream_blocks<T>(nb, sblk, rowptr, colidx, vals, x, x, prod,
-                   [&](int row, T s, T) { y[row] = s; });
+  __shared__ int tab[3][kColClusters];
+  stream_blocks<T, C16, RL>(nb, desc, rowptr, Cols{colidx, cofs}, vals, x, x, prod, tab,
+                        [&](int row, T s, T) { y[row] = s; });
 }
 
 // q = A p; partial p.q (row blocks)
-template <typename T>
+template <typename T, bool C16, int RL>
 __global__ __launch_bounds__(kThreads) void k_spmv_pq_stream(
-    int nb, const int32_t *__restrict__ sblk, const int32_t *__restrict__ rowptr,
-    const int32_t *__restrict__ colidx, const T *__restrict__ vals, const T *__restrict__ p,
+    int nb, const BlkDesc *__restrict__ desc, const int32_t *__restrict__ rowptr,
+    const int32_t *__restrict__ colidx, const uint16_t *__restrict__ cofs, const T *__restrict__ vals, const T *__restrict__ p,
     T *__restrict__ q, double *__restrict__ pq_part, const PcgState *__restrict__ st) {
   __shared__ T prod[kSpmvNnz + 1];
   __shared__ double sh[kThreads / 64];
   if (st->done) return;
   double acc = 0.0;
-  stream_blocks<T>(nb, sblk, rowptr, colidx, vals, p, p, prod, [&](int row, T s, T pr) {
+  __shared__ int tab[3][kColClusters];
+  stream_blocks<T, C16, RL>(nb, desc, rowptr, Cols{colidx, cofs}, vals, p, p, prod, tab,
+                        [&](int row, T s, T pr) {
     q[row] = s;
     acc += (double)(s * pr);
   });
@@ -229,16 +268,18 @@ __global__ __launch_bounds__(kThreads) void k_spmv_pq_stream(
 }
 
 // FP64 residual r = b - A x; partial r.r (row blocks)
+template <bool C16, int RL>
 __global__ __launch_bounds__(kThreads) void k_resid64_stream(
-    int nb, const int32_t *__restrict__ sblk, const int32_t *__restrict__ rowptr,
-    const int32_t *__restrict__ colidx, const double *__restrict__ vals,
+    int nb, const BlkDesc *__restrict__ desc, const int32_t *__restrict__ rowptr,
+    const int32_t *__restrict__ colidx, const uint16_t *__restrict__ cofs, const double *__restrict__ vals,
     const double *__restrict__ x, const double *__restrict__ b, double *__restrict__ r,
     double *__restrict__ part) {
   __shared__ double prod[kSpmvNnz + 1];
   __shared__ double sh[kThreads / 64];
   double rr = 0.0;
-  stream_blocks<double>(nb, sblk, rowptr, colidx, vals, x, b, prod,
-                        [&](int row, double ax, double br) {
+  __shared__ int tab[3][kColClusters];
+  stream_blocks<double, C16, RL>(nb, desc, rowptr, Cols{colidx, cofs}, vals, x, b, prod, tab,
+                             [&](int row, double ax, double br) {
     const double ri = br - ax;
     r[row] = ri;
     rr += ri * ri;
@@ -521,11 +562,38 @@ static int grid_blocks(const System &S) {
   return S.n_sblk < kMaxBlocks ? (S.n_sblk < 1 ? 1 : S.n_sblk) : kMaxBlocks;
 }
 
+// Calls f(C16, RL) with the system's column form and lanes per row as
+// compile-time constants.
+template <typename F>
+static void stream_variant(const System &S, F f) {
+  using T16 = std::integral_constant<bool, true>;
+  using T32 = std::integral_constant<bool, false>;
+  using L2 = std::integral_constant<int, 2>;
+  using L4 = std::integral_constant<int, 4>;
+  if (S.c16)
+    S.srl == 2 ? f(T16{}, L2{}) : f(T16{}, L4{});
+  else
+    S.srl == 2 ? f(T32{}, L2{}) : f(T32{}, L4{});
+}
+
 template <typename T>
 static void spmv_t(Ctx *c, System &S, const T *vals, const T *x, T *y) {
-  k_spmv_stream<T><<<grid_blocks(S), kThreads, 0, c->stream>>>(S.n_sblk, S.sblk, S.rowptr,
-                                                               S.colidx, vals, x, y);
+  stream_variant(S, [&](auto cf, auto rl) {
+    k_spmv_stream<T, decltype(cf)::value, decltype(rl)::value>
+        <<<grid_blocks(S), kThreads, 0, c->stream>>>(S.n_sblk, S.sdesc, S.rowptr, S.colidx, S.cofs,
+                                                     vals, x, y);
+  });
   MG_HIP(hipGetLastError());
+}
+
+// q = A p with the p.q partials, in the system's column form
+template <typename T>
+static void spmv_pq(Ctx *c, const System &S, int gb, const T *vals, const T *p, T *q,
+                    double *pq_part, const PcgState *st) {
+  stream_variant(S, [&](auto cf, auto rl) {
+    k_spmv_pq_stream<T, decltype(cf)::value, decltype(rl)::value><<<gb, kThreads, 0, c->stream>>>(
+        S.n_sblk, S.sdesc, S.rowptr, S.colidx, S.cofs, vals, p, q, pq_part, st);
+  });
 }
 
 void spmv_launch(Ctx *c, int sys, int prec, const void *x, void *y) {
@@ -636,8 +704,7 @@ static PcgState pcg_core(Ctx *c, System &S, const T *vals, const T *dinv, const 
     for (int k = 0; k < chunk; ++k, ++it) {
       double *rz_prev = (it & 1) ? P.rz0 : P.rz1;
       double *rz_new = (it & 1) ? P.rz1 : P.rz0;
-      k_spmv_pq_stream<T><<<gb, kThreads, 0, c->stream>>>(S.n_sblk, S.sblk, S.rowptr, S.colidx,
-                                                         vals, p, q, P.pq, w.st);
+      spmv_pq<T>(c, S, gb, vals, p, q, P.pq, w.st);
       k_update<T><<<gv, kThreads, 0, c->stream>>>(n, x, r, p, q, dinv, P.pq, gb, rz_prev, gv,
                                                   rz_new, P.rr, w.st);
       k_pupdate<T><<<gv, kThreads, 0, c->stream>>>(n, r, p, dinv, rz_new, rz_prev, P.rr, gv, it,
@@ -718,8 +785,7 @@ static int pcg_reliable(Ctx *c, System &S, const double *b, double *x, double bn
     for (int k = 0; k < chunk; ++k, ++it) {
       double *rz_prev = (it & 1) ? P.rz0 : P.rz1;
       double *rz_new = (it & 1) ? P.rz1 : P.rz0;
-      k_spmv_pq_stream<float><<<gb, kThreads, 0, c->stream>>>(
-          S.n_sblk, S.sblk, S.rowptr, S.colidx, S.vals32, w.p32, w.q32, P.pq, w.st);
+      spmv_pq<float>(c, S, gb, S.vals32, w.p32, w.q32, P.pq, w.st);
       k_update<float><<<gv, kThreads, 0, c->stream>>>(n, w.x32, w.r32, w.p32, w.q32, S.dinv32,
                                                       P.pq, gb, rz_prev, gv, rz_new, P.rr, w.st);
       k_pupdate<float><<<gv, kThreads, 0, c->stream>>>(n, w.r32, w.p32, S.dinv32, rz_new, rz_prev,
@@ -773,8 +839,10 @@ static double true_relres(Ctx *c, System &S, const double *b, const double *x, W
   const int n = (int)S.nrows;
   (void)n;
   const int g = grid_blocks(S);
-  k_resid64_stream<<<g, kThreads, 0, c->stream>>>(S.n_sblk, S.sblk, S.rowptr, S.colidx, S.vals64,
-                                                  x, b, w.rr64, w.part.tmp);
+  stream_variant(S, [&](auto cf, auto rl) {
+    k_resid64_stream<decltype(cf)::value, decltype(rl)::value><<<g, kThreads, 0, c->stream>>>(
+        S.n_sblk, S.sdesc, S.rowptr, S.colidx, S.cofs, S.vals64, x, b, w.rr64, w.part.tmp);
+  });
   k_finalize_sum<<<1, kThreads, 0, c->stream>>>(w.part.tmp, g, &w.st->rr);
   MG_HIP(hipGetLastError());
   PcgState h = read_state(c, w.st);

@@ -306,6 +306,55 @@ __global__ void k_inc_count(int64_t n, const uint64_t *__restrict__ keys, uint64
   if (k != invalid) atomicAdd(&cnt[k / (uint64_t)nl], 1);
 }
 
+// 16-bit column form of one SpMV row block (one workgroup per block, the
+// block's <= kSpmvNnz entries spread over 256 lanes).  Clusters are formed
+// greedily: the smallest column not yet placed opens cluster c at base_c, and
+// every unplaced column in [base_c, base_c + kColSpan) joins it as
+// c << 13 | (col - base_c).  The columns of a block of consecutive mesh rows
+// lie in a few narrow bands (the vertex and edge dofs of neighbouring mesh
+// planes), so a few clusters cover them; a block that needs more than
+// kColClusters sets *fail and the system keeps int32 columns.
+__global__ __launch_bounds__(256) void k_col_compress(BlkDesc *__restrict__ desc,
+                                                      const int32_t *__restrict__ colidx,
+                                                      uint16_t *__restrict__ ofs,
+                                                      int *__restrict__ fail) {
+  constexpr int PER = kSpmvNnz / 256;
+  __shared__ int red[4];
+  const int b = blockIdx.x;
+  const int s0 = desc[b].s0, ns = desc[b].ns;
+  int col[PER];
+  bool left[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int k = threadIdx.x + u * 256;
+    left[u] = k < ns;
+    col[u] = left[u] ? colidx[s0 + k] : 0;
+  }
+  for (int cl = 0; cl < kColClusters; ++cl) {
+    int m = INT32_MAX;
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+      if (left[u]) m = min(m, col[u]);
+    for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = min(min(red[0], red[1]), min(red[2], red[3]));
+    __syncthreads();
+    if (threadIdx.x == 0) desc[b].base[cl] = m == INT32_MAX ? 0 : m;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      if (left[u] && col[u] - m < kColSpan) {
+        ofs[s0 + threadIdx.x + u * 256] = (uint16_t)((cl << 13) | (col[u] - m));
+        left[u] = false;
+      }
+    }
+  }
+  bool any = false;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) any |= left[u];
+  if (any) *fail = 1;
+}
+
 template <int NL, typename Rec>
 __global__ void k_inc_records(int64_t n_inc, const uint64_t *__restrict__ keys,
                               const int32_t *__restrict__ vals, const int32_t *__restrict__ slot,
@@ -462,18 +511,46 @@ void build_pattern(Ctx *c, int sys) {
     std::vector<int32_t> sb;
     int64_t r = 0;
     sb.push_back(0);
+    // short rows (mean <= 16 entries, P1): 128-row blocks summed by 2 lanes per
+    // row, so a block still carries ~2048 entries (MGFE_SPMV_ROWS=64|128 overrides)
+    const int64_t mean = nrows > 0 ? S.nnz / nrows : 0;
+    const int want = env_int("MGFE_SPMV_ROWS", mean <= kSpmvNnz / (2 * kSpmvRows) ? 128 : 64);
+    const int rows_cap = want == 128 ? 2 * kSpmvRows : kSpmvRows;
+    S.srl = rows_cap == kSpmvRows ? 4 : 2;
     while (r < nrows) {
       int64_t r1 = r + 1;
       MG_CHECK(rp[r + 1] - rp[r] <= kSpmvNnz, MG_ERR_SIZE, "pattern: row longer than 2048");
-      while (r1 < nrows && r1 - r < kSpmvRows && rp[r1 + 1] - rp[r] <= kSpmvNnz) ++r1;
+      while (r1 < nrows && r1 - r < rows_cap && rp[r1 + 1] - rp[r] <= kSpmvNnz) ++r1;
       sb.push_back((int32_t)r1);
       r = r1;
     }
     S.n_sblk = (int32_t)sb.size() - 1;
-    S.sblk = dalloc<int32_t>(c, sb.size());
-    MG_HIP(hipMemcpyAsync(S.sblk, sb.data(), sizeof(int32_t) * sb.size(), hipMemcpyHostToDevice,
+    std::vector<BlkDesc> d(S.n_sblk);
+    for (int32_t b = 0; b < S.n_sblk; ++b) {
+      d[b] = BlkDesc{};
+      d[b].r0 = sb[b];
+      d[b].r1 = sb[b + 1];
+      d[b].s0 = rp[sb[b]];
+      d[b].ns = rp[sb[b + 1]] - rp[sb[b]];
+    }
+    S.sdesc = dalloc<BlkDesc>(c, d.size());
+    MG_HIP(hipMemcpyAsync(S.sdesc, d.data(), sizeof(BlkDesc) * d.size(), hipMemcpyHostToDevice,
                           c->stream));
     MG_HIP(hipStreamSynchronize(c->stream));
+  }
+  // 16-bit columns for the SpMV streams (MGFE_SPMV_COL32=1 keeps int32)
+  if (S.n_sblk > 0 && env_int("MGFE_SPMV_COL32", 0) == 0) {
+    S.cofs = dalloc<uint16_t>(c, S.nnz);
+    int *fail = dalloc<int>(c, 1);
+    MG_HIP(hipMemsetAsync(fail, 0, sizeof(int), c->stream));
+    k_col_compress<<<S.n_sblk, 256, 0, c->stream>>>(S.sdesc, S.colidx, S.cofs, fail);
+    MG_HIP(hipGetLastError());
+    S.c16 = read_back(c, fail) == 0;
+    dfree(c, fail);
+    if (!S.c16) {
+      dfree(c, S.cofs);
+      S.cofs = nullptr;
+    }
   }
   for (const RowSegment &sg : S.segs)
     MG_CHECK((sg.lds_slots + 256) * 8 <= 64 * 1024, MG_ERR_SIZE,

@@ -192,6 +192,34 @@ def _solve_case(ctx, cfg, n):
     return p, b
 
 
+@pytest.mark.parametrize("cfg,n", [(2, 12), (5, 24)])
+@pytest.mark.parametrize("rows", ["64", "128"])
+def test_spmv_column_forms_bit_identical(ctx, monkeypatch, cfg, n, rows):
+    # the 16-bit clustered column stream decodes to the int32 columns (at
+    # these sizes P2 vertex rows span several clusters: vertex and edge dofs),
+    # so SpMV and every PCG iterate agree bit for bit; both row-block shapes
+    # (64 rows x 4 lanes, 128 rows x 2 lanes) against the oracle
+    monkeypatch.setenv("MGFE_SPMV_ROWS", rows)
+    out = []
+    for col32 in ("0", "1"):
+        monkeypatch.setenv("MGFE_SPMV_COL32", col32)
+        p, b = _solve_case(ctx, cfg, n)
+        ctx.assemble(L.SYS_INTERIOR, L.TERM_STIFF, stiff_coeff=L.COEFF_P0,
+                     stiff_rule=1 if p.order == 1 else 11, prec=L.PREC_F32)
+        x = 2 * W.stream_uniform(0, 3, b.size) - 1
+        y = ctx.spmv(L.SYS_INTERIOR, x)
+        y32 = ctx.spmv(L.SYS_INTERIOR, x, prec=L.PREC_F32)
+        x64, i64 = ctx.pcg(b, mode=L.SOLVE_F64, tol=1e-10)
+        xr, ir = ctx.pcg(b, mode=L.SOLVE_F32_RELIABLE, tol=1e-10)
+        assert i64["relres"] <= 1e-10 and ir["relres"] <= 1e-10
+        out.append((y, y32, x64, xr, i64["iters"], ir["iters"]))
+    rp, ci = ctx.pattern(L.SYS_INTERIOR)
+    va = ctx.values(L.SYS_INTERIOR)
+    assert np.abs(out[0][0] - O.spmv(rp, ci, va, x)).max() <= 1e-13 * np.abs(out[0][0]).max()
+    for a, c in zip(out[0], out[1]):
+        assert np.array_equal(a, c)
+
+
 @pytest.mark.parametrize("cfg,n", [(1, 20), (2, 6), (5, 10)])
 @pytest.mark.parametrize("mode", [L.SOLVE_F64, L.SOLVE_F32_REFINE, L.SOLVE_F32_RELIABLE])
 def test_pcg_solution(ctx, cfg, n, mode):
