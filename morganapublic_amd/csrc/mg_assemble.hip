@@ -336,9 +336,9 @@ __device__ __forceinline__ void row_values(const T e[6], T md, T v[NL]) {
 // contributions in incidence order, independent of ROWS and LPR.  An
 // incidence reads its record, then only the relabelled edge values its row
 // needs (gathered by address from the element record, no selects).
-// A lane's incidences are software-pipelined: while incidence j is added, the
-// edge values of j+1 and the record of j+3 are in flight; FP32 edge rows
-// (short, about 5 incidences) load theirs in chunks instead.
+// A vertex row's incidences (about 24) are software-pipelined: while
+// incidence j is added, the edge values of j+1 and the record of j+3 are in
+// flight; edge rows (about 5 incidences) load theirs in chunks instead.
 template <int NL, int LROW, typename T, bool MASS, bool EXACT, bool ELIM, int ROWS, int LPR,
           typename Rec>
 __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
@@ -355,12 +355,11 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
   const int r1 = min(r0 + ROWS, row_end);
   const int s0 = rowptr[r0];
   const int ns = rowptr[r1] - s0;
-  // FP32 edge rows load their incidences in chunks (below) and issue the
-  // lane's row pointers before the LDS is cleared, needing them together
-  // after it (one round trip; otherwise the row offset is loaded only after
-  // the loop guard has waited for the incidence range).  Measured: FP32 edge
-  // rows -23%; the FP64 kernels are faster in the pipelined form.
-  constexpr bool kChunked = LROW == 4 && sizeof(T) == 4;
+  // Edge rows load their incidences in chunks (below) and issue the lane's
+  // row pointers before the LDS is cleared, needing them together after it
+  // (one round trip; otherwise the row offset is loaded only after the loop
+  // guard has waited for the incidence range).
+  constexpr bool kChunked = LROW == 4;
   const int r = r0 + lane / LPR;
   const int rr = kChunked ? min(r, r1 - 1) : r;
   int base = 0, iend = 0, i = 0;
@@ -418,14 +417,18 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
         idx[m] = bsel + pm;
         if (ELIM) idx[m] = pm == 0xFF ? dummy : idx[m];
       }
-      // FP64: one LDS atomic add (ds_add_f64, no return) per entry; FP32: read,
-      // add, write (ds_add_f32 is several times slower here).  The lanes of a
-      // row are in one wavefront, whose LDS operations execute in issue order;
-      // the compiler barrier keeps the phases in that order.
+      // FP64 vertex rows: one LDS atomic add (ds_add_f64, no return) per
+      // entry; otherwise read, add, write (measured: FP64 edge rows -19%,
+      // ds_add_f32 several times slower than the plain form).  In a phase
+      // every active lane owns a different row, so the plain form has no
+      // conflicts (the dummy region excepted, whose values are discarded).
+      // The lanes of a row are in one wavefront, whose LDS operations
+      // execute in issue order; the compiler barrier keeps the phases in
+      // that order.
 #pragma unroll
       for (int ph = 0; ph < LPR; ++ph) {
         if (LPR == 1 || sub == ph) {
-          if constexpr (sizeof(T) == 8) {
+          if constexpr (sizeof(T) == 8 && LROW == 0) {
 #pragma unroll
             for (int m = 0; m < NL; ++m) unsafeAtomicAdd(&acc[idx[m]], v[m]);
           } else {
