@@ -336,9 +336,10 @@ __device__ __forceinline__ void row_values(const T e[6], T md, T v[NL]) {
 // contributions in incidence order, independent of ROWS and LPR.  An
 // incidence reads its record, then only the relabelled edge values its row
 // needs (gathered by address from the element record, no selects).
-// A vertex row's incidences (about 24) are software-pipelined: while
+// An FP64 vertex row's incidences (about 24) are software-pipelined: while
 // incidence j is added, the edge values of j+1 and the record of j+3 are in
-// flight; edge rows (about 5 incidences) load theirs in chunks instead.
+// flight; edge rows (about 5 incidences) and FP32 rows load theirs in chunks
+// instead.
 template <int NL, int LROW, typename T, bool MASS, bool EXACT, bool ELIM, int ROWS, int LPR,
           typename Rec>
 __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
@@ -355,11 +356,12 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
   const int r1 = min(r0 + ROWS, row_end);
   const int s0 = rowptr[r0];
   const int ns = rowptr[r1] - s0;
-  // Edge rows load their incidences in chunks (below) and issue the lane's
-  // row pointers before the LDS is cleared, needing them together after it
-  // (one round trip; otherwise the row offset is loaded only after the loop
-  // guard has waited for the incidence range).
-  constexpr bool kChunked = LROW == 4;
+  // Edge rows and FP32 rows load their incidences in chunks (below) and
+  // issue the lane's row pointers before the LDS is cleared, needing them
+  // together after it (one round trip; otherwise the row offset is loaded
+  // only after the loop guard has waited for the incidence range).
+  // Measured at 64^3 P2: FP64 vertex rows are fastest pipelined.
+  constexpr bool kChunked = LROW == 4 || sizeof(T) == 4;
   const int r = r0 + lane / LPR;
   const int rr = kChunked ? min(r, r1 - 1) : r;
   int base = 0, iend = 0, i = 0;
@@ -443,11 +445,11 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
       }
     };
     if constexpr (kChunked) {
-      // P2 edge rows have few incidences (about 5 per row): a lane takes up to
-      // KC of them at once, records first, then all their edge values, so
-      // the whole row needs two load round trips, with nothing carried in
-      // flight from one chunk to the next.
-      constexpr int KC = LPR == 1 ? 6 : 3;
+      // A lane takes up to KC incidences at once, records first, then all
+      // their edge values, so a P2 edge row (about 5 incidences) or a vertex
+      // row (about 24, 6 per lane) needs two load round trips, with nothing
+      // carried in flight from one chunk to the next.
+      constexpr int KC = (LROW == 4 && LPR > 1) ? 3 : 6;
       for (; i < iend; i += KC * LPR) {
         Slot S[KC];
 #pragma unroll
