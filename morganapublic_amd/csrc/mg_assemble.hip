@@ -203,42 +203,60 @@ __device__ __forceinline__ T pick6(const T *s, uint32_t k) {
   return (k & 4u) ? c : ab;
 }
 
-// An incidence record read as whole dwords: el, then the pos bytes.
-template <int NL>
-struct RecWords;
-template <>
-struct RecWords<10> {
+// An incidence record as the row kernels read it (whole dwords): el, the
+// position of relabelled column m (pos), and the code of an eliminated column.
+template <typename Rec, int LROW>
+struct RecView;
+template <int LROW>
+struct RecView<RecP2, LROW> {
+  static constexpr int kElim = 0xFF;
   int32_t el;
   uint32_t w[3];
   __device__ __forceinline__ int pos(int m) const { return (w[m >> 2] >> (8 * (m & 3))) & 0xFF; }
+  __device__ __forceinline__ static RecView load(const RecP2 *__restrict__ rec, int j, int) {
+    const int4 q = reinterpret_cast<const int4 *>(rec)[j];  // el, pos[10], pad[2]
+    return RecView{q.x, {(uint32_t)q.y, (uint32_t)q.z, (uint32_t)q.w}};
+  }
 };
-template <>
-struct RecWords<4> {
+template <int LROW>
+struct RecView<RecP1, LROW> {
+  static constexpr int kElim = 0xFF;
   int32_t el;
-  uint32_t w[1];
-  __device__ __forceinline__ int pos(int m) const { return (w[0] >> (8 * m)) & 0xFF; }
+  uint32_t w;
+  __device__ __forceinline__ int pos(int m) const { return (w >> (8 * m)) & 0xFF; }
+  __device__ __forceinline__ static RecView load(const RecP1 *__restrict__ rec, int j, int) {
+    const int2 q = reinterpret_cast<const int2 *>(rec)[j];  // el, pos[4]
+    return RecView{q.x, (uint32_t)q.y};
+  }
 };
-
-template <int NL>
-__device__ __forceinline__ RecWords<NL> load_rec(const void *__restrict__ rec, int j);
-template <>
-__device__ __forceinline__ RecWords<10> load_rec<10>(const void *__restrict__ rec, int j) {
-  const int4 q = reinterpret_cast<const int4 *>(rec)[j];  // RecP2: el, pos[10], pad[2]
-  RecWords<10> r;
-  r.el = q.x;
-  r.w[0] = (uint32_t)q.y;
-  r.w[1] = (uint32_t)q.z;
-  r.w[2] = (uint32_t)q.w;
-  return r;
-}
-template <>
-__device__ __forceinline__ RecWords<4> load_rec<4>(const void *__restrict__ rec, int j) {
-  const int2 q = reinterpret_cast<const int2 *>(rec)[j];  // RecP1: el, pos[4]
-  RecWords<4> r;
-  r.el = q.x;
-  r.w[0] = (uint32_t)q.y;
-  return r;
-}
+// compact P2: the row's own column (basis LROW) is its diagonal, `self`
+template <int LROW>
+struct RecP2cView {
+  static constexpr int kElim = 0x7F;
+  int32_t el;
+  uint32_t lo, hi;
+  int self;
+  __device__ __forceinline__ int pos(int m) const {
+    if (m == LROW) return self;
+    const int k = m < LROW ? m : m - 1;  // packed index, compile-time
+    const int b = 7 * k;
+    const uint32_t v = b >= 32 ? (hi >> (b - 32)) : (b > 25 ? ((lo >> b) | (hi << (32 - b))) : (lo >> b));
+    return (int)(v & 0x7Fu);
+  }
+};
+template <int LROW>
+struct RecView<RecP2c, LROW> : RecP2cView<LROW> {
+  __device__ __forceinline__ static RecView load(const RecP2c *__restrict__ rec, int j,
+                                                 int self) {
+    const RecP2c q = rec[j];  // one dwordx3 load
+    RecView r;
+    r.el = q.el;
+    r.lo = q.lo;
+    r.hi = q.hi;
+    r.self = self;
+    return r;
+  }
+};
 
 // ---- closed form of the stiffness row for quadrature rules exact on it ------
 // With D_pq = k vol grad(lambda_p) . grad(lambda_q) of the relabelled element,
@@ -345,7 +363,8 @@ template <int NL, int LROW, typename T, bool MASS, bool EXACT, bool ELIM, int RO
 __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
     int32_t row_begin, int32_t row_end, int32_t dummy, EdgePack ep,
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ inc_ptr,
-    const Rec *__restrict__ rec, const T *__restrict__ geo, T *__restrict__ vals) {
+    const int32_t *__restrict__ diagpos, const Rec *__restrict__ rec, const T *__restrict__ geo,
+    T *__restrict__ vals) {
   extern __shared__ __align__(16) char lds_raw[];
   T *acc = reinterpret_cast<T *>(lds_raw);
   constexpr int NT = ROWS * LPR;
@@ -364,20 +383,26 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
   constexpr bool kChunked = LROW == 4 || sizeof(T) == 4;
   const int r = r0 + lane / LPR;
   const int rr = kChunked ? min(r, r1 - 1) : r;
-  int base = 0, iend = 0, i = 0;
+  // compact records leave out the row's own column: its position is the
+  // row's diagonal
+  constexpr bool kSelf = std::is_same<Rec, RecP2c>::value;
+  using RV = RecView<Rec, LROW>;
+  int base = 0, iend = 0, i = 0, self = 0;
   if (kChunked) {
     base = rowptr[rr] - s0;
     iend = inc_ptr[rr + 1];
     i = inc_ptr[rr] + sub;
+    if (kSelf) self = diagpos[rr] - s0 - base;  // position of the diagonal in the row
   }
   for (int s = lane; s < ns; s += NT) acc[s] = (T)0;
   __syncthreads();
-  if (kChunked) asm volatile("" ::"v"(base), "v"(iend), "v"(i));
+  if (kChunked) asm volatile("" ::"v"(base), "v"(iend), "v"(i), "v"(self));
   if (r < r1) {
     if (!kChunked) {
       base = rowptr[r] - s0;
       iend = inc_ptr[r + 1];
       i = inc_ptr[r] + sub;
+      if (kSelf) self = diagpos[r] - s0 - base;
     }
     // Pipelined form: three slots rotate; unrolled by three so every load
     // lands in the slot that consumes it and no loaded register is copied.
@@ -387,7 +412,7 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
     const int last = iend > i - sub ? iend - 1 : 0;  // a row without incidences reads record 0
     auto at = [&](int j) { return j < iend ? j : last; };
     struct Slot {
-      RecWords<NL> r;
+      RV r;
       T e[6];
       T md;
     };
@@ -417,7 +442,7 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
       for (int m = 0; m < NL; ++m) {
         const int pm = A.r.pos(m);
         idx[m] = bsel + pm;
-        if (ELIM) idx[m] = pm == 0xFF ? dummy : idx[m];
+        if (ELIM) idx[m] = pm == RV::kElim ? dummy : idx[m];
       }
       // FP64 vertex rows: one LDS atomic add (ds_add_f64, no return) per
       // entry; otherwise read, add, write (measured: FP64 edge rows -19%,
@@ -453,7 +478,7 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
       for (; i < iend; i += KC * LPR) {
         Slot S[KC];
 #pragma unroll
-        for (int k = 0; k < KC; ++k) S[k].r = load_rec<NL>(rec, at(i + k * LPR));
+        for (int k = 0; k < KC; ++k) S[k].r = RV::load(rec, at(i + k * LPR), self);
 #pragma unroll
         for (int k = 0; k < KC; ++k) load_vals(S[k]);
 #pragma unroll
@@ -464,11 +489,11 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
       auto step = [&](Slot &A, Slot &B, int j) {
         load_vals(B);
         add(A, j);
-        A.r = load_rec<NL>(rec, at(j + 3 * LPR));
+        A.r = RV::load(rec, at(j + 3 * LPR), self);
       };
-      S0.r = load_rec<NL>(rec, at(i));
-      S1.r = load_rec<NL>(rec, at(i + LPR));
-      S2.r = load_rec<NL>(rec, at(i + 2 * LPR));
+      S0.r = RV::load(rec, at(i), self);
+      S1.r = RV::load(rec, at(i + LPR), self);
+      S2.r = RV::load(rec, at(i + 2 * LPR), self);
       load_vals(S0);
       for (; i < iend; i += 3 * LPR) {
         step(S0, S1, i);
@@ -530,12 +555,14 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
         k_asm_rows<NL, decltype(lr)::value, T, decltype(ms)::value, decltype(ex)::value, true,
                    decltype(rw)::value, decltype(lp)::value, Rec>
             <<<nblk, decltype(rw)::value * decltype(lp)::value, lds, c->stream>>>(
-                sg.row_begin, sg.row_end, sg.lds_slots, ep, S.rowptr, S.inc_ptr, rec, geo, vals);
+                sg.row_begin, sg.row_end, sg.lds_slots, ep, S.rowptr, S.inc_ptr, S.diagpos, rec, geo,
+                vals);
       else
         k_asm_rows<NL, decltype(lr)::value, T, decltype(ms)::value, decltype(ex)::value, false,
                    decltype(rw)::value, decltype(lp)::value, Rec>
           <<<nblk, decltype(rw)::value * decltype(lp)::value, lds, c->stream>>>(
-              sg.row_begin, sg.row_end, sg.lds_slots, ep, S.rowptr, S.inc_ptr, rec, geo, vals);
+              sg.row_begin, sg.row_end, sg.lds_slots, ep, S.rowptr, S.inc_ptr, S.diagpos, rec, geo,
+                vals);
     };
     // vertex-dof rows: 64 rows per workgroup, 1/2/4 lanes per row; P2 edge-dof
     // rows: 64/128/256 rows per workgroup, 1/2 lanes per row
@@ -579,7 +606,8 @@ static void assemble_t(Ctx *c, int sys, const mg_operator *op, int prec) {
   upload_tables(c, c->order, stiff ? op->stiff_rule : 0, mass ? op->mass_rule : 0, prec);
   T *vals = prec == MG_PREC_F64 ? (T *)S.vals64 : (T *)S.vals32;
   if (c->nloc == 10)
-    launch_rows<10, T, RecP2>(c, S, op, vals, sys == MG_SYS_INTERIOR);
+    S.rec_words == 3 ? launch_rows<10, T, RecP2c>(c, S, op, vals, sys == MG_SYS_INTERIOR)
+                     : launch_rows<10, T, RecP2>(c, S, op, vals, sys == MG_SYS_INTERIOR);
   else
     launch_rows<4, T, RecP1>(c, S, op, vals, sys == MG_SYS_INTERIOR);
 }

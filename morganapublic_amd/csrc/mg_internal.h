@@ -26,6 +26,14 @@ struct alignas(8) RecP1 {
   int32_t el;
   uint8_t pos[4];
 };
+// Compact P2 record (12 bytes, used when every row has <= 127 entries): the
+// row's own column (relabelled basis 0 for a vertex row, 4 for an edge row)
+// is the row's diagonal and is not stored; the other 9 positions take 7 bits
+// each, in increasing relabelled-basis order (0x7F: eliminated column).
+struct alignas(4) RecP2c {
+  int32_t el;
+  uint32_t lo, hi;  // 63 bits of positions
+};
 
 // Rows of one assembly launch: blocks of `rows` rows, each staging its CSR
 // entries in LDS; lrow is the local basis every incidence of these rows is relabelled to
@@ -59,7 +67,8 @@ struct System {
   int32_t *diagpos = nullptr;  // nrows, CSR index of the diagonal entry
   int32_t *slot = nullptr;     // nT * nloc * nloc element -> CSR slot map
   int32_t *inc_ptr = nullptr;  // nrows + 1 (row -> incidences)
-  void *inc_rec = nullptr;     // n_inc records (RecP1 or RecP2), sorted by (row, l, e)
+  void *inc_rec = nullptr;     // n_inc records (RecP1, RecP2 or RecP2c), sorted by (row, l, e)
+  int rec_words = 0;           // int32 words per record: 2 (RecP1), 4 (RecP2), 3 (RecP2c)
   std::vector<RowSegment> segs;
   // SpMV row blocks: each holds <= kSpmvNnz entries and <= kSpmvRows rows
   BlkDesc *sdesc = nullptr;

@@ -106,7 +106,7 @@ static int64_t load_len(Ctx *c, int kind) {
 static void load_launch(Ctx *c, int sys, int kind, const double *f, int rule, double *F) {
   System &S = c->sys[sys];
   const LoadTables tab = load_tables(c->order, rule);
-  const int rs = c->nloc == 10 ? 4 : 2;
+  const int rs = S.rec_words;
   const int blocks = (int)std::min<int64_t>(blocks_for(S.nrows, 256), 4096);
   if (S.nrows == 0) return;
   if (c->nloc == 10)

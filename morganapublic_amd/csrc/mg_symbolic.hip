@@ -382,6 +382,41 @@ __global__ void k_inc_records(int64_t n_inc, const uint64_t *__restrict__ keys,
   rec[i] = r;
 }
 
+// Compact P2 records: the 9 positions other than the row's own column, 7 bits
+// each; the own column (relabelled basis 0 or 4) must be the row's diagonal.
+__global__ void k_inc_records_c(int64_t n_inc, const uint64_t *__restrict__ keys,
+                                const int32_t *__restrict__ vals,
+                                const int32_t *__restrict__ slot,
+                                const int32_t *__restrict__ rowptr,
+                                const int32_t *__restrict__ diagpos, RecP2c *__restrict__ rec,
+                                int *__restrict__ bad) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_inc) return;
+  const int32_t v = vals[i];
+  const int64_t e = v >> 4;
+  const int l = v & 15;
+  const int lrow = l < 4 ? 0 : 4;
+  const int64_t row = (int64_t)(keys[i] / 10);
+  const int32_t base = rowptr[row];
+  uint64_t w = 0;
+  int k = 0;
+  for (int m = 0; m < 10; ++m) {
+    const int32_t s = slot[e * 100 + c_sigma[l][m] * 10 + l];
+    if (m == lrow) {
+      if (s != diagpos[row]) atomicAdd(bad, 1);
+      continue;
+    }
+    int p = s < 0 ? 0x7F : s - base;
+    if (s >= 0 && (p < 0 || p >= 0x7F)) {
+      atomicAdd(bad, 1);
+      p = 0x7F;
+    }
+    w |= (uint64_t)p << (7 * k);
+    ++k;
+  }
+  rec[i] = RecP2c{v, (uint32_t)w, (uint32_t)(w >> 32)};
+}
+
 // Assembly launches: the vertex-dof rows (relabelled local basis 0) and the
 // edge-dof rows (basis 4) of a P2 system go to separate launches; each launch
 // uses 64-row blocks and the LDS size of its largest block.
@@ -481,12 +516,29 @@ void build_pattern(Ctx *c, int sys) {
                                   hipMemcpyHostToDevice, c->stream));
     MG_HIP(hipStreamSynchronize(c->stream));
   }
-  if (nl == 10) {
+  std::vector<int32_t> rp(nrows + 1);
+  MG_HIP(hipMemcpyAsync(rp.data(), S.rowptr, sizeof(int32_t) * (nrows + 1), hipMemcpyDeviceToHost,
+                        c->stream));
+  MG_HIP(hipStreamSynchronize(c->stream));
+  int maxnnz = 0;
+  for (int64_t r = 0; r < nrows; ++r) maxnnz = std::max(maxnnz, rp[r + 1] - rp[r]);
+  S.max_row_nnz = maxnnz;
+  // P2: 12-byte records when every row has <= 127 entries (MGFE_REC16=1
+  // keeps the 16-byte form)
+  const bool compact = nl == 10 && maxnnz <= 127 && env_int("MGFE_REC16", 0) == 0;
+  if (compact) {
+    S.inc_rec = dalloc<RecP2c>(c, S.n_inc);
+    S.rec_words = 3;
+    k_inc_records_c<<<blocks_for(S.n_inc, 256), 256, 0, c->stream>>>(
+        S.n_inc, k1, v1, S.slot, S.rowptr, S.diagpos, (RecP2c *)S.inc_rec, bad);
+  } else if (nl == 10) {
     S.inc_rec = dalloc<RecP2>(c, S.n_inc);
+    S.rec_words = 4;
     k_inc_records<10, RecP2><<<blocks_for(S.n_inc, 256), 256, 0, c->stream>>>(
         S.n_inc, k1, v1, S.slot, S.rowptr, (RecP2 *)S.inc_rec, bad);
   } else {
     S.inc_rec = dalloc<RecP1>(c, S.n_inc);
+    S.rec_words = 2;
     k_inc_records<4, RecP1><<<blocks_for(S.n_inc, 256), 256, 0, c->stream>>>(
         S.n_inc, k1, v1, S.slot, S.rowptr, (RecP1 *)S.inc_rec, bad);
   }
@@ -495,14 +547,6 @@ void build_pattern(Ctx *c, int sys) {
   for (void *p : {(void *)k0, (void *)k1, (void *)v0, (void *)v1, (void *)cnt, (void *)bad})
     dfree(c, p);
   MG_CHECK(hb == 0, MG_ERR_SIZE, "pattern: a row has more than 254 entries (uint8 positions)");
-
-  std::vector<int32_t> rp(nrows + 1);
-  MG_HIP(hipMemcpyAsync(rp.data(), S.rowptr, sizeof(int32_t) * (nrows + 1), hipMemcpyDeviceToHost,
-                        c->stream));
-  MG_HIP(hipStreamSynchronize(c->stream));
-  int maxnnz = 0;
-  for (int64_t r = 0; r < nrows; ++r) maxnnz = std::max(maxnnz, rp[r + 1] - rp[r]);
-  S.max_row_nnz = maxnnz;
   const int64_t nvrows =
       nl == 4 ? nrows : (sys == MG_SYS_FULL ? c->nV : c->nfree_vertex);
   make_segments(S, rp, nvrows);

@@ -137,26 +137,32 @@ def test_assembly_deterministic(ctx):
     assert all(np.array_equal(runs[0], r) for r in runs[1:])
 
 
-SHAPES = [("1", "64", "1"), ("2", "128", "2"), ("4", "256", "1"), ("4", "256", "2")]
+# (vertex lanes per row, edge rows per workgroup, edge lanes per row, 16-byte
+# P2 records instead of the 12-byte form)
+SHAPES = [("1", "64", "1", "0"), ("2", "128", "2", "0"), ("4", "256", "1", "0"),
+          ("4", "256", "2", "0"), ("4", "256", "2", "1"), ("1", "64", "1", "1")]
 
 
 @pytest.mark.parametrize("order,rs", [(1, 1), (2, 11), (2, 15)])
 @pytest.mark.parametrize("prec", [L.PREC_F64, L.PREC_F32])
-def test_assembly_launch_shapes_bit_identical(ctx, monkeypatch, order, rs, prec):
-    # lanes per row and rows per workgroup change the schedule, never the
-    # order in which a row receives its incidences
+@pytest.mark.parametrize("sysid", [L.SYS_FULL, L.SYS_INTERIOR])
+def test_assembly_launch_shapes_bit_identical(ctx, monkeypatch, order, rs, prec, sysid):
+    # lanes per row, rows per workgroup and the record form change the
+    # schedule and the bytes read, never the order in which a row receives
+    # its incidences
     m = W.kuhn_cube(7, jitter=0.2, seed=5)
     k = np.exp(W.stream_normal(0, W.STREAM_K, m.nT))
     out = []
-    for vl, er, el in SHAPES:
+    for vl, er, el, r16 in SHAPES:
         monkeypatch.setenv("MGFE_VERTEX_LPR", vl)
         monkeypatch.setenv("MGFE_EDGE_ROWS", er)
         monkeypatch.setenv("MGFE_EDGE_LPR", el)
+        monkeypatch.setenv("MGFE_REC16", r16)
         setup(ctx, m, order)
         ctx.field_set(L.FIELD_STIFF, k)
-        ctx.assemble(L.SYS_FULL, L.TERM_STIFF | L.TERM_MASS, stiff_coeff=L.COEFF_P0,
+        ctx.assemble(sysid, L.TERM_STIFF | L.TERM_MASS, stiff_coeff=L.COEFF_P0,
                      stiff_rule=rs, mass_rule=15, prec=prec)
-        out.append(ctx.values(L.SYS_FULL, prec=prec))
+        out.append(ctx.values(sysid, prec=prec))
     assert all(np.array_equal(out[0], o) for o in out[1:])
 
 
