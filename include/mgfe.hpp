@@ -16,6 +16,8 @@
  *   factoryPreconditioner::create (.hpp:53-87)            factoryPreconditioner::create
  *   factoryLinearSolver::create (.hpp:51-91)              factoryLinearSolver::create
  *   virtualLinearSolver::setProblem/setLeftPrec/solve     virtualLinearSolver (same methods)
+ *   tpetraBlockMatrixAssemble::printToFile (.cpp:470-475) CrsMatrix::printToFile (MatrixMarket)
+ *   epetraBlockMatrixAssemble::printToFile (.cpp:412-416) CrsMatrix::printToMatlabFile
  *
  * Plain std::vector containers stand in for Epetra's, and failures throw
  * std::runtime_error with mg_last_error's text instead of asserting.  One
@@ -25,6 +27,7 @@
 #define MGFE_HPP
 
 #include <cstdint>
+#include <cstdio>
 #include <fstream>
 #include <map>
 #include <memory>
@@ -219,6 +222,32 @@ struct CrsMatrix {
   std::vector<double> values;
   int64_t NumGlobalRows() const { return (int64_t)rowptr.size() - 1; }
   int64_t NumGlobalNonzeros() const { return (int64_t)colidx.size(); }
+
+  // MatrixMarket coordinate file (real general, 1-based, every stored entry
+  // including structural zeros), as Tpetra's MatrixMarket writer that
+  // tpetraBlockMatrixAssemble::printToFile calls; values with 17 significant
+  // digits, so they read back exactly
+  void printToFile(const std::string &name) const {
+    write(name, true);
+  }
+  // "row col value" lines, 1-based (Matlab's spconvert form), as
+  // EpetraExt::RowMatrixToMatlabFile that epetraBlockMatrixAssemble::printToFile
+  // calls
+  void printToMatlabFile(const std::string &name) const { write(name, false); }
+
+ private:
+  void write(const std::string &name, bool market) const {
+    std::FILE *f = std::fopen(name.c_str(), "w");
+    if (!f) throw std::runtime_error("CrsMatrix: cannot open " + name);
+    const int64_t n = NumGlobalRows();
+    if (market)
+      std::fprintf(f, "%%%%MatrixMarket matrix coordinate real general\n%lld %lld %lld\n",
+                   (long long)n, (long long)n, (long long)NumGlobalNonzeros());
+    for (int64_t r = 0; r < n; ++r)
+      for (int32_t k = rowptr[r]; k < rowptr[r + 1]; ++k)
+        std::fprintf(f, "%lld %d %.17g\n", (long long)(r + 1), colidx[k] + 1, values[k]);
+    if (std::fclose(f) != 0) throw std::runtime_error("CrsMatrix: cannot write " + name);
+  }
 };
 
 // matrixBuilder over localMatrixEL3d: intPoints = quadrature points of the

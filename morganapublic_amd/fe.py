@@ -159,6 +159,27 @@ class CrsMatrix:
         n = self.NumGlobalRows()
         return sp.csr_matrix((self.values, self.colidx, self.rowptr), shape=(n, n))
 
+    def _write(self, name: str, market: bool):
+        n = self.NumGlobalRows()
+        rows = np.repeat(np.arange(1, n + 1), np.diff(self.rowptr))
+        with open(name, "w") as f:
+            if market:
+                f.write("%%%%MatrixMarket matrix coordinate real general\n%d %d %d\n"
+                        % (n, n, self.NumGlobalNonzeros()))
+            for r, c, v in zip(rows, self.colidx + 1, self.values):
+                f.write("%d %d %s\n" % (r, c, repr(float(v))))
+
+    def printToFile(self, name: str):
+        """MatrixMarket coordinate file (real general, 1-based, structural zeros
+        kept), as tpetraBlockMatrixAssemble::printToFile
+        (tpetraBlockMatrixAssemble.cpp:470-475); values read back exactly."""
+        self._write(name, True)
+
+    def printToMatlabFile(self, name: str):
+        """'row col value' lines, 1-based, as epetraBlockMatrixAssemble::printToFile
+        (epetraBlockMatrixAssemble.cpp:412-416, EpetraExt::RowMatrixToMatlabFile)."""
+        self._write(name, False)
+
 
 class MatrixBuilder:
     """matrixBuilder over localMatrixEL3d<FIELD,TEST,...,INT_PRECISION>.

@@ -46,6 +46,8 @@ int main(int argc, char **argv) {
     std::fwrite(matrix.values.data(), sizeof(double), matrix.values.size(), out);
     std::fwrite(F.data(), sizeof(double), F.size(), out);
     std::fclose(out);
+    matrix.printToFile(std::string(argv[4]) + ".mtx");
+    matrix.printToMatlabFile(std::string(argv[4]) + ".m");
     std::printf("rows %lld nnz %lld\n", (long long)head[0], (long long)head[1]);
   } catch (const std::exception &e) {
     std::fprintf(stderr, "%s\n", e.what());

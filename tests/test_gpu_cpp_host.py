@@ -57,6 +57,16 @@ def test_matrix_and_vector_builders(tmp_path, order, points, mesh):
     assert np.abs(va - ova).max() <= 1e-12 * np.abs(ova).max()
     Fo = O.load_vector(order, m, O.LOAD_P0, k, points)
     assert np.abs(F - Fo).max() <= 1e-13 * np.abs(Fo).max()
+    # printToFile (MatrixMarket) and printToMatlabFile read back exactly
+    import scipy.io
+    n = rp.size - 1
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    mm = scipy.io.mmread(out + ".mtx").tocsr()
+    assert mm.nnz == ci.size and (mm != A).nnz == 0
+    t = np.loadtxt(out + ".m")
+    np.testing.assert_array_equal(t[:, 0].astype(np.int64) - 1, np.repeat(np.arange(n), np.diff(rp)))
+    np.testing.assert_array_equal(t[:, 1].astype(np.int64) - 1, ci)
+    np.testing.assert_array_equal(t[:, 2], va)
 
 
 @pytest.mark.parametrize("precision", ["double", "mixed", "float+refine"])
