@@ -16,6 +16,8 @@
 namespace mg {
 
 __constant__ int c_ev[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+// the same table for compile-time indexing in unrolled loops
+constexpr int c_ev_h[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
 __constant__ int c_fv[4][3] = {{0, 2, 1}, {0, 1, 3}, {1, 2, 3}, {0, 3, 2}};  // faceToPoint
 __constant__ int c_fe[4][3] = {{2, 1, 0}, {0, 4, 3}, {1, 5, 4}, {3, 5, 2}};  // faceToEdge
 
@@ -179,11 +181,13 @@ void build_dual(Ctx *c) {
   int32_t *eptr = nullptr, *einc = nullptr;
   group_by(c, c->elem_edge, 6 * nT, nE, 0, &eptr, &einc);
   build_vertex_incidence(c);
-  for (double *p : {c->edge_surf, c->edge_n, c->cell_volume, c->sigma_el}) dfree(c, p);
+  for (double *p : {c->edge_surf, c->edge_n, c->cell_volume, c->sigma_el, c->dd_contrib})
+    dfree(c, p);
   c->edge_surf = dalloc<double>(c, nE);
   c->edge_n = dalloc<double>(c, 3 * nE);
   c->cell_volume = dalloc<double>(c, nV);
   c->sigma_el = dalloc<double>(c, 6 * nT);
+  c->dd_contrib = dalloc<double>(c, 8 * nT);
   k_edge_gather<<<blocks_for(nE, 256), 256, 0, c->stream>>>(nE, eptr, einc, ne, c->edge_surf,
                                                           c->edge_n);
   k_sigma<<<blocks_for(6 * nT, 256), 256, 0, c->stream>>>(nT, c->elem_edge, ne, c->edge_n,
@@ -256,36 +260,70 @@ __device__ __forceinline__ double sg_edge(const double *__restrict__ xyz, const 
   return coef / L * (dens[b] * bernoulli(dab) - dens[a] * bernoulli(-dab));
 }
 
-// vertex-owner residual: each vertex gathers the contributions of its elements
-// in incidence order
-__global__ void k_dd_residual(int64_t nV, const int32_t *__restrict__ vptr,
-                              const int32_t *__restrict__ vinc, const int32_t *__restrict__ tets,
-                              const double *__restrict__ xyz, const double *__restrict__ sigma,
-                              const double *__restrict__ eps, const double *__restrict__ rho,
-                              const double *__restrict__ mu, const double *__restrict__ phi,
-                              const double *__restrict__ dens, const uint8_t *__restrict__ fixed,
-                              double *__restrict__ rphi, double *__restrict__ rn) {
-  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nV) return;
-  double sp = 0.0, sn = 0.0;
-  for (int k = vptr[v]; k < vptr[v + 1]; ++k) {
-    const int32_t el = vinc[k];
-    const int64_t e = el >> 2;
-    const int l = el & 3;
-    int32_t id[4];
-    for (int q = 0; q < 4; ++q) id[q] = tets[4 * e + q];
-    double G[4][3];
-    const double vol = p1_grads(xyz, id, G);
-    double gp[3] = {0, 0, 0};
-    for (int q = 0; q < 4; ++q)
-      for (int d = 0; d < 3; ++d) gp[d] += phi[id[q]] * G[q][d];
-    sp += eps[e] * vol * (G[l][0] * gp[0] + G[l][1] * gp[1] + G[l][2] * gp[2]) - rho[e] * vol / 4.0;
+// The residual in two passes.  k_dd_elem: one lane per element computes its
+// geometry, grad(phi_h) and the SG fluxes of its 6 edges once, and writes, for
+// each local vertex l, the Poisson term eps vol grad(lambda_l).grad(phi_h) -
+// rho vol / 4 and the SG term sum over the edges j at l (in j order) of
+// +-F_j (F_j oriented from the edge's first to its second local vertex), as
+// one double2 at index 4e + l = the incidence code e << 2 | l.
+// k_dd_gather: one lane per vertex sums its incidences' terms in incidence
+// order (deterministic, no atomics).
+__global__ __launch_bounds__(128) void k_dd_elem(
+    int64_t nT, const int32_t *__restrict__ tets, const double *__restrict__ xyz,
+    const double *__restrict__ sigma, const double *__restrict__ eps,
+    const double *__restrict__ rho, const double *__restrict__ mu, const double *__restrict__ phi,
+    const double *__restrict__ dens, double2 *__restrict__ contrib) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nT) return;
+  const int4 t = reinterpret_cast<const int4 *>(tets)[e];
+  const int32_t id[4] = {t.x, t.y, t.z, t.w};
+  double G[4][3];
+  const double vol = p1_grads(xyz, id, G);
+  double gp[3] = {0, 0, 0};
+  for (int q = 0; q < 4; ++q)
+    for (int d = 0; d < 3; ++d) gp[d] += phi[id[q]] * G[q][d];
+  const double ee = eps[e], re = rho[e], me = mu[e];
+  double F[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) F[j] = sg_edge(xyz, id, j, me * sigma[6 * e + j], phi, dens);
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const double cp = ee * vol * (G[l][0] * gp[0] + G[l][1] * gp[1] + G[l][2] * gp[2]) -
+                      re * vol / 4.0;
+    double cn = 0.0;
+#pragma unroll
     for (int j = 0; j < 6; ++j) {
-      const bool first = c_ev[j][0] == l, second = c_ev[j][1] == l;
-      if (!first && !second) continue;
-      const double F = sg_edge(xyz, id, j, mu[e] * sigma[6 * e + j], phi, dens);
-      sn += first ? F : -F;
+      if (c_ev_h[j][0] == l) cn += F[j];
+      if (c_ev_h[j][1] == l) cn -= F[j];
     }
+    contrib[4 * e + l] = make_double2(cp, cn);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dd_gather(int64_t nV, const int32_t *__restrict__ vptr,
+                                                   const int32_t *__restrict__ vinc,
+                                                   const double2 *__restrict__ contrib,
+                                                   const uint8_t *__restrict__ fixed,
+                                                   double *__restrict__ rphi,
+                                                   double *__restrict__ rn) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nV) return;
+  constexpr int U = 4;  // incidences in flight per lane
+  const int k0 = vptr[v], k1 = vptr[v + 1];
+  double sp = 0.0, sn = 0.0;
+  for (int k = k0; k < k1; k += U) {
+    int el[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) el[u] = vinc[k + u < k1 ? k + u : k0];
+    double2 c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = contrib[el[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (k + u < k1) {
+        sp += c[u].x;
+        sn += c[u].y;
+      }
   }
   const bool fx = fixed[v] != 0;
   rphi[v] = fx ? 0.0 : sp;
@@ -380,9 +418,12 @@ static void dd_free(Ctx *c, DDBuffers &B) {
 }
 
 static void dd_residual_launch(Ctx *c, DDBuffers &B) {
-  k_dd_residual<<<blocks_for(c->nV, 128), 128, 0, c->stream>>>(
-      c->nV, c->vinc_ptr, c->vinc, c->tets, c->xyz, c->sigma_el, B.eps, B.rho, B.mu, B.phi, B.dens,
-      B.fixed, B.rphi, B.rn);
+  double2 *contrib = reinterpret_cast<double2 *>(c->dd_contrib);
+  k_dd_elem<<<blocks_for(c->nT, 128), 128, 0, c->stream>>>(c->nT, c->tets, c->xyz, c->sigma_el,
+                                                          B.eps, B.rho, B.mu, B.phi, B.dens,
+                                                          contrib);
+  k_dd_gather<<<blocks_for(c->nV, 256), 256, 0, c->stream>>>(c->nV, c->vinc_ptr, c->vinc, contrib,
+                                                            B.fixed, B.rphi, B.rn);
   MG_HIP(hipGetLastError());
 }
 
