@@ -251,7 +251,8 @@ static void reset_derived(Ctx *c) {
   for (void *p : {(void *)c->edge_keys, (void *)c->edge_nodes, (void *)c->elem_edge,
                   (void *)c->dofs, (void *)c->fixed, (void *)c->sysmap, (void *)c->edge_surf,
                   (void *)c->edge_n, (void *)c->cell_volume, (void *)c->sigma_el,
-                  (void *)c->vinc_ptr, (void *)c->vinc, (void *)c->dd_contrib})
+                  (void *)c->vinc_ptr, (void *)c->vinc, (void *)c->vinc_pos,
+                  (void *)c->dd_contrib})
     dfree(c, p);
   c->edge_keys = nullptr;
   c->edge_nodes = c->elem_edge = c->dofs = c->sysmap = nullptr;
@@ -259,6 +260,7 @@ static void reset_derived(Ctx *c) {
   c->edge_surf = c->edge_n = c->cell_volume = c->sigma_el = nullptr;
   c->vinc_ptr = c->vinc = nullptr;
   c->dd_contrib = nullptr;
+  c->vinc_pos = nullptr;
   c->edges_ok = c->dirichlet_ok = c->dual_ok = false;
   c->order = c->nloc = 0;
   c->nE = c->ndofs = c->nfree = 0;

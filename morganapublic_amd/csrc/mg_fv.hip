@@ -16,8 +16,6 @@
 namespace mg {
 
 __constant__ int c_ev[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
-// the same table for compile-time indexing in unrolled loops
-constexpr int c_ev_h[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
 __constant__ int c_fv[4][3] = {{0, 2, 1}, {0, 1, 3}, {1, 2, 3}, {0, 3, 2}};  // faceToPoint
 __constant__ int c_fe[4][3] = {{2, 1, 0}, {0, 4, 3}, {1, 5, 4}, {3, 5, 2}};  // faceToEdge
 
@@ -167,9 +165,17 @@ static void group_by(Ctx *c, const int32_t *keys_src, int64_t n, int64_t nkeys, 
   *inc_out = v1;
 }
 
+__global__ void k_inverse(int64_t n, const int32_t *__restrict__ perm, int32_t *__restrict__ inv) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) inv[perm[k]] = (int32_t)k;
+}
+
 static void build_vertex_incidence(Ctx *c) {
   if (c->vinc_ptr) return;
   group_by(c, c->tets, 4 * c->nT, c->nV, 1, &c->vinc_ptr, &c->vinc);
+  c->vinc_pos = dalloc<int32_t>(c, 4 * c->nT);
+  k_inverse<<<blocks_for(4 * c->nT, 256), 256, 0, c->stream>>>(4 * c->nT, c->vinc, c->vinc_pos);
+  MG_HIP(hipGetLastError());
 }
 
 void build_dual(Ctx *c) {
@@ -227,15 +233,16 @@ __device__ __forceinline__ double p1_grads(const double *__restrict__ xyz, const
   const double d = T[0][0] * T[1][1] * T[2][2] - T[0][0] * T[1][2] * T[2][1] -
                    T[1][0] * T[0][1] * T[2][2] + T[1][0] * T[0][2] * T[2][1] +
                    T[2][0] * T[0][1] * T[1][2] - T[2][0] * T[0][2] * T[1][1];
-  G[1][0] = (T[1][1] * T[2][2] - T[1][2] * T[2][1]) / d;
-  G[1][1] = -(T[0][1] * T[2][2] - T[0][2] * T[2][1]) / d;
-  G[1][2] = (T[0][1] * T[1][2] - T[0][2] * T[1][1]) / d;
-  G[2][0] = -(T[1][0] * T[2][2] - T[1][2] * T[2][0]) / d;
-  G[2][1] = (T[0][0] * T[2][2] - T[0][2] * T[2][0]) / d;
-  G[2][2] = -(T[0][0] * T[1][2] - T[0][2] * T[1][0]) / d;
-  G[3][0] = (T[1][0] * T[2][1] - T[1][1] * T[2][0]) / d;
-  G[3][1] = -(T[0][0] * T[2][1] - T[0][1] * T[2][0]) / d;
-  G[3][2] = (T[0][0] * T[1][1] - T[0][1] * T[1][0]) / d;
+  const double r = 1.0 / d;  // one division, nine products
+  G[1][0] = (T[1][1] * T[2][2] - T[1][2] * T[2][1]) * r;
+  G[1][1] = -(T[0][1] * T[2][2] - T[0][2] * T[2][1]) * r;
+  G[1][2] = (T[0][1] * T[1][2] - T[0][2] * T[1][1]) * r;
+  G[2][0] = -(T[1][0] * T[2][2] - T[1][2] * T[2][0]) * r;
+  G[2][1] = (T[0][0] * T[2][2] - T[0][2] * T[2][0]) * r;
+  G[2][2] = -(T[0][0] * T[1][2] - T[0][2] * T[1][0]) * r;
+  G[3][0] = (T[1][0] * T[2][1] - T[1][1] * T[2][0]) * r;
+  G[3][1] = -(T[0][0] * T[2][1] - T[0][1] * T[2][0]) * r;
+  G[3][2] = (T[0][0] * T[1][1] - T[0][1] * T[1][0]) * r;
   for (int k = 0; k < 3; ++k) G[0][k] = -(G[1][k] + G[2][k] + G[3][k]);
   return d / 6.0;
 }
@@ -250,14 +257,25 @@ __device__ __forceinline__ double edge_len(const double *__restrict__ xyz, int a
 }
 
 // SG flux of local edge j of element e, oriented from its first to its second
-// local vertex: mu sigma / L * (n_b B(phi_b - phi_a) - n_a B(phi_a - phi_b))
-__device__ __forceinline__ double sg_edge(const double *__restrict__ xyz, const int32_t *id, int j,
+// local vertex: mu sigma / L * (n_b B(phi_b - phi_a) - n_a B(phi_a - phi_b)).
+// One Bernoulli evaluation per edge: with t = |x|, B(-t) = t + B(t) (both
+// terms positive, no cancellation).
+__device__ __forceinline__ double sg_flux(const double *__restrict__ xyz, int a, int b,
                                           double coef, const double *__restrict__ phi,
                                           const double *__restrict__ dens) {
-  const int a = id[c_ev[j][0]], b = id[c_ev[j][1]];
   const double L = edge_len(xyz, a, b);
   const double dab = phi[b] - phi[a];
-  return coef / L * (dens[b] * bernoulli(dab) - dens[a] * bernoulli(-dab));
+  const double t = fabs(dab);
+  const double bt = bernoulli(t), bmt = t + bt;  // B(t), B(-t)
+  const double bp = dab >= 0.0 ? bt : bmt;       // B(dab)
+  const double bm = dab >= 0.0 ? bmt : bt;       // B(-dab)
+  return coef / L * (dens[b] * bp - dens[a] * bm);
+}
+
+// vertex id of local vertex q (q uniform, id in registers: selects, no
+// indexed register access)
+__device__ __forceinline__ int pick4(const int32_t *id, int q) {
+  return q == 0 ? id[0] : q == 1 ? id[1] : q == 2 ? id[2] : id[3];
 }
 
 // The residual in two passes.  k_dd_elem: one lane per element computes its
@@ -265,16 +283,21 @@ __device__ __forceinline__ double sg_edge(const double *__restrict__ xyz, const 
 // each local vertex l, the Poisson term eps vol grad(lambda_l).grad(phi_h) -
 // rho vol / 4 and the SG term sum over the edges j at l (in j order) of
 // +-F_j (F_j oriented from the edge's first to its second local vertex), as
-// one double2 at index 4e + l = the incidence code e << 2 | l.
-// k_dd_gather: one lane per vertex sums its incidences' terms in incidence
-// order (deterministic, no atomics).
+// one double2 at the position of the incidence (e, l) in the vertex
+// incidence list (vinc_pos), so the scattered accesses are writes.
+// k_dd_gather: one lane per vertex sums its incidences' terms, a contiguous
+// range, in incidence order (deterministic, no atomics), and the workgroup
+// writes its partial of the loss sum (r_phi^2 + r_n^2).
 __global__ __launch_bounds__(128) void k_dd_elem(
     int64_t nT, const int32_t *__restrict__ tets, const double *__restrict__ xyz,
     const double *__restrict__ sigma, const double *__restrict__ eps,
     const double *__restrict__ rho, const double *__restrict__ mu, const double *__restrict__ phi,
-    const double *__restrict__ dens, double2 *__restrict__ contrib) {
+    const double *__restrict__ dens, const int32_t *__restrict__ pos,
+    double2 *__restrict__ contrib) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nT) return;
+  const int4 ps = reinterpret_cast<const int4 *>(pos)[e];
+  const int32_t at[4] = {ps.x, ps.y, ps.z, ps.w};
   const int4 t = reinterpret_cast<const int4 *>(tets)[e];
   const int32_t id[4] = {t.x, t.y, t.z, t.w};
   double G[4][3];
@@ -283,51 +306,66 @@ __global__ __launch_bounds__(128) void k_dd_elem(
   for (int q = 0; q < 4; ++q)
     for (int d = 0; d < 3; ++d) gp[d] += phi[id[q]] * G[q][d];
   const double ee = eps[e], re = rho[e], me = mu[e];
-  double F[6];
+  // the 6 edges one at a time (a rolled loop keeps the register count, and
+  // with it the occupancy, of the latency-bound gathers low); each local
+  // vertex receives +-F_j in j order
+  double cn[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int j = 0; j < 6; ++j) {
+    const int qa = c_ev[j][0], qb = c_ev[j][1];
+    const double F = sg_flux(xyz, pick4(id, qa), pick4(id, qb), me * sigma[6 * e + j], phi, dens);
 #pragma unroll
-  for (int j = 0; j < 6; ++j) F[j] = sg_edge(xyz, id, j, me * sigma[6 * e + j], phi, dens);
+    for (int l = 0; l < 4; ++l) {
+      if (qa == l) cn[l] += F;
+      if (qb == l) cn[l] -= F;
+    }
+  }
 #pragma unroll
   for (int l = 0; l < 4; ++l) {
     const double cp = ee * vol * (G[l][0] * gp[0] + G[l][1] * gp[1] + G[l][2] * gp[2]) -
                       re * vol / 4.0;
-    double cn = 0.0;
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      if (c_ev_h[j][0] == l) cn += F[j];
-      if (c_ev_h[j][1] == l) cn -= F[j];
-    }
-    contrib[4 * e + l] = make_double2(cp, cn);
+    contrib[at[l]] = make_double2(cp, cn[l]);
   }
 }
 
 __global__ __launch_bounds__(256) void k_dd_gather(int64_t nV, const int32_t *__restrict__ vptr,
-                                                   const int32_t *__restrict__ vinc,
                                                    const double2 *__restrict__ contrib,
                                                    const uint8_t *__restrict__ fixed,
                                                    double *__restrict__ rphi,
-                                                   double *__restrict__ rn) {
+                                                   double *__restrict__ rn,
+                                                   double *__restrict__ partial) {
+  __shared__ double sh[4];
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nV) return;
-  constexpr int U = 4;  // incidences in flight per lane
-  const int k0 = vptr[v], k1 = vptr[v + 1];
-  double sp = 0.0, sn = 0.0;
-  for (int k = k0; k < k1; k += U) {
-    int el[U];
+  double sq = 0.0;
+  if (v < nV) {
+    constexpr int U = 8;  // loads in flight per lane (addresses known up front)
+    const int k0 = vptr[v], k1 = vptr[v + 1];
+    double sp = 0.0, sn = 0.0;
+    for (int k = k0; k < k1; k += U) {
+      double2 c[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) el[u] = vinc[k + u < k1 ? k + u : k0];
-    double2 c[U];
+      for (int u = 0; u < U; ++u) c[u] = contrib[k + u < k1 ? k + u : k0];
 #pragma unroll
-    for (int u = 0; u < U; ++u) c[u] = contrib[el[u]];
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (k + u < k1) {
-        sp += c[u].x;
-        sn += c[u].y;
-      }
+      for (int u = 0; u < U; ++u)
+        if (k + u < k1) {
+          sp += c[u].x;
+          sn += c[u].y;
+        }
+    }
+    const bool fx = fixed[v] != 0;
+    sp = fx ? 0.0 : sp;
+    sn = fx ? 0.0 : sn;
+    rphi[v] = sp;
+    rn[v] = sn;
+    sq = sp * sp + sn * sn;
   }
-  const bool fx = fixed[v] != 0;
-  rphi[v] = fx ? 0.0 : sp;
-  rn[v] = fx ? 0.0 : sn;
+  if (partial) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_down(sq, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  }
 }
 
 __global__ void k_dd_vjp(int64_t nT, const int32_t *__restrict__ tets,
@@ -337,8 +375,8 @@ __global__ void k_dd_vjp(int64_t nT, const int32_t *__restrict__ tets,
                          double *__restrict__ g_eps, double *__restrict__ g_mu) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nT) return;
-  int32_t id[4];
-  for (int q = 0; q < 4; ++q) id[q] = tets[4 * e + q];
+  const int4 t = reinterpret_cast<const int4 *>(tets)[e];
+  const int32_t id[4] = {t.x, t.y, t.z, t.w};
   double G[4][3];
   const double vol = p1_grads(xyz, id, G);
   double gp[3] = {0, 0, 0};
@@ -350,36 +388,29 @@ __global__ void k_dd_vjp(int64_t nT, const int32_t *__restrict__ tets,
   double gm = 0.0;
   for (int j = 0; j < 6; ++j) {
     const int a = id[c_ev[j][0]], b = id[c_ev[j][1]];
-    const double f = sg_edge(xyz, id, j, sigma[6 * e + j], phi, dens);
+    const double f = sg_flux(xyz, a, b, sigma[6 * e + j], phi, dens);
     gm += 2.0 * (rn[a] - rn[b]) * f;
   }
   g_eps[e] = ge;
   g_mu[e] = gm;
 }
 
-__global__ void k_loss(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
-                       double *__restrict__ partial) {
+// total of the n block partials in a fixed order (one workgroup of 256: lane t
+// sums partials t, t + 256, ... in order, then a fixed shuffle tree), the same
+// for every entry point
+__global__ __launch_bounds__(256) void k_sum_fixed(int n, const double *__restrict__ partial,
+                                                   double *__restrict__ out) {
   __shared__ double sh[4];
   double s = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    s += a[i] * a[i] + b[i] * b[i];
+  for (int i = threadIdx.x; i < n; i += 256) s += partial[i];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+  if (threadIdx.x == 0) *out = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
-// sum of the block partials in index order on one lane (the same order the
-// host-side loss used), so both entry points return the same bits
-__global__ void k_sum_in_order(int n, const double *__restrict__ partial, double *__restrict__ out) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < n; ++i) s += partial[i];
-    *out = s;
-  }
-}
+static int dd_gather_blocks(const Ctx *c) { return blocks_for(c->nV, 256); }
 
 struct DDBuffers {
   double *eps, *rho, *mu, *phi, *dens, *rphi, *rn, *ge, *gm, *partial;
@@ -399,7 +430,7 @@ static DDBuffers dd_upload(Ctx *c, const double *eps, const double *rho, const d
   B.dens = dalloc<double>(c, nV);
   B.rphi = dalloc<double>(c, nV);
   B.rn = dalloc<double>(c, nV);
-  B.partial = dalloc<double>(c, 1025);
+  B.partial = dalloc<double>(c, dd_gather_blocks(c) + 1);
   B.fixed = dalloc<uint8_t>(c, nV);
   MG_HIP(hipMemcpyAsync(B.eps, eps, sizeof(double) * nT, hipMemcpyHostToDevice, c->stream));
   MG_HIP(hipMemcpyAsync(B.rho, rho, sizeof(double) * nT, hipMemcpyHostToDevice, c->stream));
@@ -417,13 +448,17 @@ static void dd_free(Ctx *c, DDBuffers &B) {
     dfree(c, p);
 }
 
-static void dd_residual_launch(Ctx *c, DDBuffers &B) {
+// residual; with loss != nullptr also the loss sum(r_phi^2 + r_n^2), whose
+// block partials go to B.partial (dd_gather_blocks entries)
+static void dd_residual_launch(Ctx *c, DDBuffers &B, double *loss = nullptr) {
   double2 *contrib = reinterpret_cast<double2 *>(c->dd_contrib);
   k_dd_elem<<<blocks_for(c->nT, 128), 128, 0, c->stream>>>(c->nT, c->tets, c->xyz, c->sigma_el,
                                                           B.eps, B.rho, B.mu, B.phi, B.dens,
-                                                          contrib);
-  k_dd_gather<<<blocks_for(c->nV, 256), 256, 0, c->stream>>>(c->nV, c->vinc_ptr, c->vinc, contrib,
-                                                            B.fixed, B.rphi, B.rn);
+                                                          c->vinc_pos, contrib);
+  const int gb = dd_gather_blocks(c);
+  k_dd_gather<<<gb, 256, 0, c->stream>>>(c->nV, c->vinc_ptr, contrib, B.fixed, B.rphi, B.rn,
+                                         loss ? B.partial : nullptr);
+  if (loss) k_sum_fixed<<<1, 256, 0, c->stream>>>(gb, B.partial, loss);
   MG_HIP(hipGetLastError());
 }
 
@@ -482,18 +517,14 @@ int mg_dd_residual(mg_ctx *c, const double *eps, const double *rho, const double
   MG_CHECK(eps && rho && mu && phi && dens && fixed, MG_ERR_ARG, "dd: null input");
   MG_CHECK(c->dual_ok, MG_ERR_STATE, "dd: dual data not built");
   DDBuffers B = dd_upload(c, eps, rho, mu, phi, dens, fixed);
-  dd_residual_launch(c, B);
+  double *dloss = B.partial + dd_gather_blocks(c);
+  dd_residual_launch(c, B, loss ? dloss : nullptr);
   if (r_phi)
     MG_HIP(hipMemcpyAsync(r_phi, B.rphi, sizeof(double) * c->nV, hipMemcpyDeviceToHost, c->stream));
   if (r_n)
     MG_HIP(hipMemcpyAsync(r_n, B.rn, sizeof(double) * c->nV, hipMemcpyDeviceToHost, c->stream));
-  if (loss) {
-    k_loss<<<1024, 256, 0, c->stream>>>(c->nV, B.rphi, B.rn, B.partial);
-    k_sum_in_order<<<1, 64, 0, c->stream>>>(1024, B.partial, B.partial + 1024);
-    MG_HIP(hipGetLastError());
-    MG_HIP(hipMemcpyAsync(loss, B.partial + 1024, sizeof(double), hipMemcpyDeviceToHost,
-                          c->stream));
-  }
+  if (loss)
+    MG_HIP(hipMemcpyAsync(loss, dloss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   MG_HIP(hipStreamSynchronize(c->stream));
   dd_free(c, B);
   API_END(c)
@@ -528,13 +559,8 @@ int mg_dd_residual_device(mg_ctx *c, const double *eps, const double *rho, const
   DDBuffers B{const_cast<double *>(eps), const_cast<double *>(rho), const_cast<double *>(mu),
               const_cast<double *>(phi), const_cast<double *>(dens), r_phi, r_n, nullptr, nullptr,
               nullptr, const_cast<uint8_t *>(fixed)};
-  dd_residual_launch(c, B);
-  if (loss) {
-    B.partial = (double *)scratch(c, sizeof(double) * 1024);
-    k_loss<<<1024, 256, 0, c->stream>>>(c->nV, B.rphi, B.rn, B.partial);
-    k_sum_in_order<<<1, 64, 0, c->stream>>>(1024, B.partial, loss);
-    MG_HIP(hipGetLastError());
-  }
+  if (loss) B.partial = (double *)scratch(c, sizeof(double) * dd_gather_blocks(c));
+  dd_residual_launch(c, B, loss);
   API_END(c)
 }
 

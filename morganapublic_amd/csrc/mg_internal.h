@@ -134,7 +134,8 @@ struct Ctx {
   double *edge_surf = nullptr, *edge_n = nullptr, *cell_volume = nullptr, *sigma_el = nullptr;
   int32_t *vinc_ptr = nullptr;  // vertex -> (element, local vertex) incidence
   int32_t *vinc = nullptr;
-  double *dd_contrib = nullptr;  // 8 nT: drift-diffusion residual terms per (element, local vertex)
+  int32_t *vinc_pos = nullptr;   // 4 nT: incidence (4e + l) -> its index in vinc
+  double *dd_contrib = nullptr;  // 8 nT: drift-diffusion residual terms in vinc order  // 8 nT: drift-diffusion residual terms per (element, local vertex)
 
   // scratch for library primitives (sorts, scans)
   void *tmp = nullptr;
