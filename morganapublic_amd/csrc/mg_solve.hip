@@ -18,6 +18,8 @@
 // state every `check_every` iterations; once converged, queued kernels return
 // at entry.  No reduction uses float atomics.
 #include <cmath>
+#include <cstdint>
+#include <initializer_list>
 #include <type_traits>
 
 #include "mg_internal.h"
@@ -288,8 +290,23 @@ __global__ __launch_bounds__(kThreads) void k_resid64_stream(
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// alpha = rz / pq; x += alpha p; r -= alpha q; partials r.D^{-1}r and r.r
+// The elementwise passes read 16-byte vectors (VEC: every array 16-byte
+// aligned), 8 entries per array per thread and round, all loads of a round
+// issued before any use; the n % (16 / sizeof(T)) last entries go to the
+// scalar path of block 0.
 template <typename T>
+struct V16;
+template <>
+struct V16<float> {
+  using type = float4;
+};
+template <>
+struct V16<double> {
+  using type = double2;
+};
+
+// alpha = rz / pq; x += alpha p; r -= alpha q; partials r.D^{-1}r and r.r
+template <typename T, bool VEC>
 __global__ __launch_bounds__(kThreads) void k_update(int n, T *__restrict__ x, T *__restrict__ r,
                                                      const T *__restrict__ p,
                                                      const T *__restrict__ q,
@@ -313,12 +330,52 @@ __global__ __launch_bounds__(kThreads) void k_update(int n, T *__restrict__ x, T
   }
   const T alpha = (T)(rz / pq);
   double a1 = 0.0, a2 = 0.0;
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+  auto one = [&](int i) {
     x[i] += alpha * p[i];
     const T ri = r[i] - alpha * q[i];
     r[i] = ri;
     a1 += (double)(ri * (dinv[i] * ri));
     a2 += (double)(ri * ri);
+  };
+  if constexpr (VEC) {
+    using VT = typename V16<T>::type;
+    constexpr int VW = 16 / sizeof(T), U = 8 / VW;
+    const int nv = n / VW;
+    VT *xv = reinterpret_cast<VT *>(x), *rv = reinterpret_cast<VT *>(r);
+    const VT *pv = reinterpret_cast<const VT *>(p), *qv = reinterpret_cast<const VT *>(q),
+             *dv = reinterpret_cast<const VT *>(dinv);
+    for (int j0 = blockIdx.x * kThreads * U + threadIdx.x; j0 < nv; j0 += gridDim.x * kThreads * U) {
+      VT X[U], R[U], P[U], Q[U], D[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = min(j0 + u * kThreads, nv - 1);
+        X[u] = xv[j];
+        R[u] = rv[j];
+        P[u] = pv[j];
+        Q[u] = qv[j];
+        D[u] = dv[j];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (j0 + u * kThreads >= nv) continue;
+        T *xe = reinterpret_cast<T *>(&X[u]), *re = reinterpret_cast<T *>(&R[u]);
+        const T *pe = reinterpret_cast<const T *>(&P[u]), *qe = reinterpret_cast<const T *>(&Q[u]),
+                *de = reinterpret_cast<const T *>(&D[u]);
+#pragma unroll
+        for (int k = 0; k < VW; ++k) {
+          xe[k] += alpha * pe[k];
+          const T ri = re[k] - alpha * qe[k];
+          re[k] = ri;
+          a1 += (double)(ri * (de[k] * ri));
+          a2 += (double)(ri * ri);
+        }
+        xv[j0 + u * kThreads] = X[u];
+        rv[j0 + u * kThreads] = R[u];
+      }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < n - nv * VW) one(nv * VW + threadIdx.x);
+  } else {
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) one(i);
   }
   const double b1 = block_sum_all(a1, sh);
   const double b2 = block_sum_all(a2, sh);
@@ -329,7 +386,7 @@ __global__ __launch_bounds__(kThreads) void k_update(int n, T *__restrict__ x, T
 }
 
 // convergence test on r.r; beta = rz_new / rz; p = D^{-1} r + beta p
-template <typename T>
+template <typename T, bool VEC>
 __global__ __launch_bounds__(kThreads) void k_pupdate(int n, const T *__restrict__ r,
                                                       T *__restrict__ p,
                                                       const T *__restrict__ dinv,
@@ -370,8 +427,39 @@ __global__ __launch_bounds__(kThreads) void k_pupdate(int n, const T *__restrict
     return;
   }
   const T beta = (T)(rz_new / rz);
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
-    p[i] = dinv[i] * r[i] + beta * p[i];
+  if constexpr (VEC) {
+    using VT = typename V16<T>::type;
+    constexpr int VW = 16 / sizeof(T), U = 8 / VW;
+    const int nv = n / VW;
+    VT *pv = reinterpret_cast<VT *>(p);
+    const VT *rv = reinterpret_cast<const VT *>(r), *dv = reinterpret_cast<const VT *>(dinv);
+    for (int j0 = blockIdx.x * kThreads * U + threadIdx.x; j0 < nv; j0 += gridDim.x * kThreads * U) {
+      VT P[U], R[U], D[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = min(j0 + u * kThreads, nv - 1);
+        P[u] = pv[j];
+        R[u] = rv[j];
+        D[u] = dv[j];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (j0 + u * kThreads >= nv) continue;
+        T *pe = reinterpret_cast<T *>(&P[u]);
+        const T *re = reinterpret_cast<const T *>(&R[u]), *de = reinterpret_cast<const T *>(&D[u]);
+#pragma unroll
+        for (int k = 0; k < VW; ++k) pe[k] = de[k] * re[k] + beta * pe[k];
+        pv[j0 + u * kThreads] = P[u];
+      }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < n - nv * VW) {
+      const int i = nv * VW + threadIdx.x;
+      p[i] = dinv[i] * r[i] + beta * p[i];
+    }
+  } else {
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
+      p[i] = dinv[i] * r[i] + beta * p[i];
+  }
 }
 
 // r = b - A x (or r = b when x is null), p = D^{-1} r; partials r.p, r.r
@@ -610,6 +698,37 @@ __global__ void k_vals_to_f32(int64_t n, const double *__restrict__ a, float *__
   if (i < n) b[i] = (float)a[i];
 }
 
+static bool aligned16(std::initializer_list<const void *> ps) {
+  for (const void *q : ps)
+    if (reinterpret_cast<uintptr_t>(q) & 15) return false;
+  return true;
+}
+
+// k_update / k_pupdate with 16-byte vector access when every array allows it
+template <typename T>
+static void vec_update(Ctx *c, int grid, int n, T *x, T *r, const T *p, const T *q, const T *dinv,
+                       const double *pq_part, int npq, const double *rz_prev, int nprev,
+                       double *rz_out, double *rr_out, PcgState *st) {
+  if (aligned16({x, r, p, q, dinv}))
+    k_update<T, true><<<grid, kThreads, 0, c->stream>>>(n, x, r, p, q, dinv, pq_part, npq, rz_prev,
+                                                        nprev, rz_out, rr_out, st);
+  else
+    k_update<T, false><<<grid, kThreads, 0, c->stream>>>(n, x, r, p, q, dinv, pq_part, npq,
+                                                         rz_prev, nprev, rz_out, rr_out, st);
+}
+
+template <typename T>
+static void vec_pupdate(Ctx *c, int grid, int n, const T *r, T *p, const T *dinv,
+                        const double *rz_new_part, const double *rz_part, const double *rr_part,
+                        int np, int iter, PcgState *st) {
+  if (aligned16({r, p, dinv}))
+    k_pupdate<T, true><<<grid, kThreads, 0, c->stream>>>(n, r, p, dinv, rz_new_part, rz_part,
+                                                         rr_part, np, iter, st);
+  else
+    k_pupdate<T, false><<<grid, kThreads, 0, c->stream>>>(n, r, p, dinv, rz_new_part, rz_part,
+                                                          rr_part, np, iter, st);
+}
+
 struct Workspace {
   double *x64, *r64;
   float *b32, *x32, *r32, *p32, *q32;
@@ -705,9 +824,9 @@ static PcgState pcg_core(Ctx *c, System &S, const T *vals, const T *dinv, const 
       double *rz_prev = (it & 1) ? P.rz0 : P.rz1;
       double *rz_new = (it & 1) ? P.rz1 : P.rz0;
       spmv_pq<T>(c, S, gb, vals, p, q, P.pq, w.st);
-      k_update<T><<<gv, kThreads, 0, c->stream>>>(n, x, r, p, q, dinv, P.pq, gb, rz_prev, gv,
+      vec_update<T>(c, gv, n, x, r, p, q, dinv, P.pq, gb, rz_prev, gv,
                                                   rz_new, P.rr, w.st);
-      k_pupdate<T><<<gv, kThreads, 0, c->stream>>>(n, r, p, dinv, rz_new, rz_prev, P.rr, gv, it,
+      vec_pupdate<T>(c, gv, n, r, p, dinv, rz_new, rz_prev, P.rr, gv, it,
                                                    w.st);
     }
     MG_HIP(hipGetLastError());
@@ -786,9 +905,9 @@ static int pcg_reliable(Ctx *c, System &S, const double *b, double *x, double bn
       double *rz_prev = (it & 1) ? P.rz0 : P.rz1;
       double *rz_new = (it & 1) ? P.rz1 : P.rz0;
       spmv_pq<float>(c, S, gb, S.vals32, w.p32, w.q32, P.pq, w.st);
-      k_update<float><<<gv, kThreads, 0, c->stream>>>(n, w.x32, w.r32, w.p32, w.q32, S.dinv32,
+      vec_update<float>(c, gv, n, w.x32, w.r32, w.p32, w.q32, S.dinv32,
                                                       P.pq, gb, rz_prev, gv, rz_new, P.rr, w.st);
-      k_pupdate<float><<<gv, kThreads, 0, c->stream>>>(n, w.r32, w.p32, S.dinv32, rz_new, rz_prev,
+      vec_pupdate<float>(c, gv, n, w.r32, w.p32, S.dinv32, rz_new, rz_prev,
                                                        P.rr, gv, it, w.st);
     }
     MG_HIP(hipGetLastError());
