@@ -103,6 +103,12 @@ def bytes_pcg_iter(n, nnz, vb):
     return nnz * (vb + 4) + 4 * (n + 1) + 13 * vb * n
 
 
+def bytes_spmv(n, nnz, vb):
+    """Algorithmic bytes of y = A x: values and 4-byte columns once, row
+    pointers, x and y once."""
+    return nnz * (vb + 4) + 4 * (n + 1) + 2 * vb * n
+
+
 def measured_traffic(n):
     """HBM bytes of one FP64 assembly at this size from the committed counter
     summary (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the copy kernel's
@@ -338,6 +344,23 @@ def main():
         b = np.ascontiguousarray(p.f[fixed == 0])
         db = timer.to_device(b)
         dx = timer.zeros(8 * nint)
+        # SpMV alone on the interior system (the PCG's operator), both precisions
+        spmv = {}
+        xs = {L.PREC_F64: db, L.PREC_F32: timer.zeros(4 * nint)}
+        ys = {L.PREC_F64: dx, L.PREC_F32: timer.zeros(4 * nint)}
+        for name, prec, vb in (("f64", L.PREC_F64, 8), ("f32", L.PREC_F32, 4)):
+            ctx.spmv_device(L.SYS_INTERIOR, xs[prec], ys[prec], prec=prec)
+            timer.sync()
+            timer.start()
+            for _ in range(20):
+                ctx.spmv_device(L.SYS_INTERIOR, xs[prec], ys[prec], prec=prec)
+            us = timer.stop_ms() / 20 * 1e3
+            gbs = bytes_spmv(nint, nnz_int, vb) / (us * 1e-6) / 1e9
+            spmv[name] = {"us": round(us, 1), "GBps": round(gbs, 1),
+                          "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
+            log("SpMV %s (interior): %.1f us, %.0f GB/s on algorithmic bytes" % (name, us, gbs))
+        timer.free(xs[L.PREC_F32])
+        timer.free(ys[L.PREC_F32])
         res = {}
         modes = (("f64", L.SOLVE_F64), ("f32_refine", L.SOLVE_F32_REFINE),
                  ("f32_reliable", L.SOLVE_F32_RELIABLE))
@@ -380,6 +403,7 @@ def main():
         it_bytes64 = bytes_pcg_iter(nint, nnz_int, 8)
         it_bytes32 = bytes_pcg_iter(nint, nnz_int, 4)
         out_extra = {
+            "spmv_interior": spmv,
             "pcg_to_1e-10": res,
             "pcg_mixed_method": mixed,
             "pcg_speedup_mixed_vs_f64": round(res["f64"]["ms"] / res[mixed]["ms"], 3),

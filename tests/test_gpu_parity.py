@@ -454,3 +454,26 @@ def test_mixed_solve_follows_fp64_reassembly(ctx, mode):
     x, info = ctx.pcg(b, mode=mode, tol=1e-10)
     assert info["relres"] <= 1e-10
     assert np.linalg.norm(x - x64) <= 1e-8 * np.linalg.norm(x64)
+
+
+@pytest.mark.parametrize("mode", [L.SOLVE_F64, L.SOLVE_F32_RELIABLE])
+def test_pcg_device_unaligned_vectors(ctx, mode):
+    # caller buffers 8 bytes off a 16-byte boundary take the scalar path of
+    # the elementwise kernels; both paths reach the same solution
+    torch = pytest.importorskip("torch")
+    p, b = _solve_case(ctx, 2, 8)
+    n = b.size
+    out = []
+    for off in (0, 1):
+        db = torch.zeros(n + 2, dtype=torch.float64, device="cuda")
+        dx = torch.zeros(n + 2, dtype=torch.float64, device="cuda")
+        db[off:off + n] = torch.from_numpy(b).cuda()
+        info = ctx.pcg_device(db.data_ptr() + 8 * off, dx.data_ptr() + 8 * off, mode=mode,
+                              tol=1e-10)
+        assert info["status"] == L.MG_OK and info["relres"] <= 1e-10, info
+        out.append(dx[off:off + n].cpu().numpy())
+    rp, ci = ctx.pattern(L.SYS_INTERIOR)
+    A = sp.csr_matrix((ctx.values(L.SYS_INTERIOR), ci, rp), shape=(n, n))
+    for x in out:
+        assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) <= 1.5e-10
+    assert np.linalg.norm(out[0] - out[1]) <= 1e-8 * np.linalg.norm(out[0])
