@@ -1,23 +1,23 @@
 // Numeric assembly of P1/P2 tetrahedral operators into a fixed CSR pattern,
 // and the adjoint element kernel for d(loss)/d(per-element coefficient).
 //
-// One kernel per assembly launch segment (k_asm_rows): one lane owns one CSR
-// row and walks the (element, local basis l) incidences of that row, built
-// from the element -> CSR slot map.  Each element is relabelled by an even
-// vertex permutation so that the row's dof is local basis 0 (vertex) or 4
-// (edge); the quadrature rules are symmetric, so the relabelled element matrix
-// is the same matrix, and row 0 / row 4 of it is
-//   sum_k c det (J^{-1} J^{-T})_k tab[k][0 or 4][m]
-// with J, det J (signed, as geoMapInterface::getGradientDet) and the adjugate
-// computed in FP64 from the vertex coordinates, and tab = the quadrature of
+// An assembly is three launches.  k_elem_factors: one lane per element
+// computes, in FP64 from the vertex coordinates, the 6 edge factors
+// S_ab = c vol grad(lambda_a).grad(lambda_b) (and c_mass det with a mass
+// term), stored per element.  k_asm_rows, once over the vertex-dof rows and
+// once over the P2 edge-dof rows: a workgroup owns consecutive CSR rows and
+// stages their value range in LDS; LPR lanes per row walk the row's
+// (element, local basis l) incidence records, whose element is relabelled by
+// an even vertex permutation so that the row's dof is local basis 0 (vertex)
+// or 4 (edge).  The quadrature rules are symmetric, so the relabelled element
+// matrix is the same matrix, and its row 0 / row 4 is a fixed linear map of
+// the edge factors: the closed form of the barycentric integrals when the
+// rule is exact on the integrand (row_values), otherwise the quadrature of
 // integratorOEL3d_STD contracted once on the reference element (constant
-// memory, compile-time indices -> scalar loads).  When the rule is exact on
-// the integrand, the same row comes from its closed form in barycentric
-// gradients (exact_row), about a third of the arithmetic.  The contributions go into the
-// row's entries staged in LDS; a row belongs to one lane and its incidences
-// are visited in a fixed order, so the sums are deterministic without atomics.
-// The workgroup then writes its contiguous CSR value range with coalesced
-// stores.
+// memory, compile-time indices).  The lanes of a row add their element rows
+// in ordered phases, so every entry sums its contributions in incidence order
+// (deterministic, no global atomics), and the workgroup writes its contiguous
+// value range with coalesced nontemporal stores.
 #include <cstring>
 #include <type_traits>
 

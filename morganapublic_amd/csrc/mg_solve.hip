@@ -98,10 +98,10 @@ __device__ __forceinline__ T row_dot(int row, int lane, const int32_t *__restric
 }
 
 // CSR-stream product of row blocks: a block's entries are streamed by all
-// 256 threads (coalesced) into LDS as vals * x[col], then 4 lanes per row sum
-// their row in a fixed order.  The (column, value) loads of the workgroup's
-// next block are issued before the current block is consumed (two register
-// buffers), so the stream never waits on its own latency.
+// 256 threads (coalesced) into LDS as vals * x[col], then RL lanes per row
+// (4, or 2 for 128-row blocks) sum their row in a fixed order.  Three blocks
+// are in flight per workgroup (stream_blocks), so the stream never waits on
+// loads issued in the same step.
 //
 // Columns come from colidx (int32) or, when every row block fits
 // kColClusters clusters (C16), from the 16-bit form: entry = cluster << 13 |
