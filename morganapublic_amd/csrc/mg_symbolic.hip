@@ -1,5 +1,6 @@
 // Symbolic phase on the GPU: edge numbering, dof map, Dirichlet numbering,
-// CSR pattern, element -> CSR slot map and row-incidence records.
+// CSR pattern, element -> CSR slot map, row-incidence records (12- or
+// 16-byte P2 form), SpMV row-block descriptors and the 16-bit column form.
 //
 // Everything is sort-based (rocPRIM radix sort through hipCUB) so it scales to
 // the 10^8 (row, column) pairs of a 64^3 P2 mesh without host work, and every
