@@ -381,6 +381,23 @@ def main():
         # the faster FP32 + FP64-refinement method is the mixed solver of record
         mixed = min(("f32_refine", "f32_reliable"), key=lambda k: res[k]["ms"])
         mmode = dict(modes)[mixed]
+        # config 3 errors (outside the timed regions): FP32 matrix against the
+        # FP64 one (the parity tests pin FP64 to the oracle), and the mixed
+        # solution against the FP64 solution
+        rp_i, _ = ctx.pattern(L.SYS_INTERIOR)
+        v64, v32 = ctx.values(L.SYS_INTERIOR), ctx.values(L.SYS_INTERIOR, prec=L.PREC_F32)
+        rows = np.repeat(np.arange(nint), np.diff(rp_i))
+        rmax = np.zeros(nint)
+        np.maximum.at(rmax, rows, np.abs(v64))
+        mat_err = float((np.abs(v32.astype(np.float64) - v64) / rmax[rows]).max())
+        del v64, v32, rows
+        ctx.pcg_device(db, dx, mode=L.SOLVE_F64, tol=1e-10)
+        x64 = timer.to_host(dx, nint)
+        ctx.pcg_device(db, dx, mode=mmode, tol=1e-10)
+        x32 = timer.to_host(dx, nint)
+        sol_err = float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
+        log("config 3: FP32 matrix max row-scaled error %.2e, mixed solution vs FP64 %.2e"
+            % (mat_err, sol_err))
         # gradient / forward cost: forward = assemble + mixed solve; gradient =
         # adjoint mixed solve + adjoint element kernel (dJ/dk, J = ||u - u_obs||^2)
         uobs = (2 * W.stream_uniform(0, W.STREAM_UOBS, fixed.size) - 1)[fixed == 0]
@@ -404,6 +421,8 @@ def main():
         it_bytes32 = bytes_pcg_iter(nint, nnz_int, 4)
         out_extra = {
             "spmv_interior": spmv,
+            "config3_errors": {"fp32_matrix_rowscaled_max": mat_err,
+                               "mixed_solution_rel_vs_fp64": sol_err},
             "pcg_to_1e-10": res,
             "pcg_mixed_method": mixed,
             "pcg_speedup_mixed_vs_f64": round(res["f64"]["ms"] / res[mixed]["ms"], 3),
