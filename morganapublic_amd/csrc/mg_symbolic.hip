@@ -558,15 +558,16 @@ void build_pattern(Ctx *c, int sys) {
     std::vector<int32_t> sb;
     int64_t r = 0;
     sb.push_back(0);
-    // short rows (mean <= 16 entries, P1): 128-row blocks summed by 2 lanes per
-    // row, so a block still carries ~2048 entries (MGFE_SPMV_ROWS=64|128 overrides)
+    // very short rows (mean <= kSpmvNnz / 128 entries): 128-row blocks summed
+    // by 2 lanes per row, so a block still carries ~kSpmvNnz entries
+    // (MGFE_SPMV_ROWS=64|128 overrides)
     const int64_t mean = nrows > 0 ? S.nnz / nrows : 0;
     const int want = env_int("MGFE_SPMV_ROWS", mean <= kSpmvNnz / (2 * kSpmvRows) ? 128 : 64);
     const int rows_cap = want == 128 ? 2 * kSpmvRows : kSpmvRows;
     S.srl = rows_cap == kSpmvRows ? 4 : 2;
     while (r < nrows) {
       int64_t r1 = r + 1;
-      MG_CHECK(rp[r + 1] - rp[r] <= kSpmvNnz, MG_ERR_SIZE, "pattern: row longer than 2048");
+      MG_CHECK(rp[r + 1] - rp[r] <= kSpmvNnz, MG_ERR_SIZE, "pattern: a row has more than 1024 entries (SpMV block)");
       while (r1 < nrows && r1 - r < rows_cap && rp[r1 + 1] - rp[r] <= kSpmvNnz) ++r1;
       sb.push_back((int32_t)r1);
       r = r1;
