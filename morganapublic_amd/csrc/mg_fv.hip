@@ -16,6 +16,8 @@
 namespace mg {
 
 __constant__ int c_ev[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+// the same edge table for compile-time indexing (local edge j = (kEA[j], kEB[j]))
+constexpr int kEA[6] = {0, 1, 2, 0, 1, 2}, kEB[6] = {1, 2, 0, 3, 3, 3};
 __constant__ int c_fv[4][3] = {{0, 2, 1}, {0, 1, 3}, {1, 2, 3}, {0, 3, 2}};  // faceToPoint
 __constant__ int c_fe[4][3] = {{2, 1, 0}, {0, 4, 3}, {1, 5, 4}, {3, 5, 2}};  // faceToEdge
 
@@ -224,58 +226,52 @@ __global__ void k_bernoulli(int64_t n, const double *__restrict__ x, double *__r
   if (i < n) y[i] = bernoulli(x[i]);
 }
 
-// P1 geometry: grad lambda (4x3) and volume
-__device__ __forceinline__ double p1_grads(const double *__restrict__ xyz, const int32_t *id,
-                                           double G[4][3]) {
+// An element's vertex coordinates, and the signed det of J with the
+// barycentric gradients G (row q = grad lambda_q), from the coordinates
+// (one division).
+__device__ __forceinline__ double p1_geometry(const double P[4][3], double G[4][3]) {
   double T[3][3];
+#pragma unroll
   for (int cc = 0; cc < 3; ++cc)
-    for (int r = 0; r < 3; ++r) T[r][cc] = xyz[3 * id[cc + 1] + r] - xyz[3 * id[0] + r];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) T[r][cc] = P[cc + 1][r] - P[0][r];
   const double d = T[0][0] * T[1][1] * T[2][2] - T[0][0] * T[1][2] * T[2][1] -
                    T[1][0] * T[0][1] * T[2][2] + T[1][0] * T[0][2] * T[2][1] +
                    T[2][0] * T[0][1] * T[1][2] - T[2][0] * T[0][2] * T[1][1];
-  const double r = 1.0 / d;  // one division, nine products
-  G[1][0] = (T[1][1] * T[2][2] - T[1][2] * T[2][1]) * r;
-  G[1][1] = -(T[0][1] * T[2][2] - T[0][2] * T[2][1]) * r;
-  G[1][2] = (T[0][1] * T[1][2] - T[0][2] * T[1][1]) * r;
-  G[2][0] = -(T[1][0] * T[2][2] - T[1][2] * T[2][0]) * r;
-  G[2][1] = (T[0][0] * T[2][2] - T[0][2] * T[2][0]) * r;
-  G[2][2] = -(T[0][0] * T[1][2] - T[0][2] * T[1][0]) * r;
-  G[3][0] = (T[1][0] * T[2][1] - T[1][1] * T[2][0]) * r;
-  G[3][1] = -(T[0][0] * T[2][1] - T[0][1] * T[2][0]) * r;
-  G[3][2] = (T[0][0] * T[1][1] - T[0][1] * T[1][0]) * r;
+  const double rd = 1.0 / d;
+  G[1][0] = (T[1][1] * T[2][2] - T[1][2] * T[2][1]) * rd;
+  G[1][1] = -(T[0][1] * T[2][2] - T[0][2] * T[2][1]) * rd;
+  G[1][2] = (T[0][1] * T[1][2] - T[0][2] * T[1][1]) * rd;
+  G[2][0] = -(T[1][0] * T[2][2] - T[1][2] * T[2][0]) * rd;
+  G[2][1] = (T[0][0] * T[2][2] - T[0][2] * T[2][0]) * rd;
+  G[2][2] = -(T[0][0] * T[1][2] - T[0][2] * T[1][0]) * rd;
+  G[3][0] = (T[1][0] * T[2][1] - T[1][1] * T[2][0]) * rd;
+  G[3][1] = -(T[0][0] * T[2][1] - T[0][1] * T[2][0]) * rd;
+  G[3][2] = (T[0][0] * T[1][1] - T[0][1] * T[1][0]) * rd;
+#pragma unroll
   for (int k = 0; k < 3; ++k) G[0][k] = -(G[1][k] + G[2][k] + G[3][k]);
-  return d / 6.0;
+  return d;
 }
 
-__device__ __forceinline__ double edge_len(const double *__restrict__ xyz, int a, int b) {
-  double s = 0.0;
-  for (int d = 0; d < 3; ++d) {
-    const double t = xyz[3 * a + d] - xyz[3 * b + d];
-    s += t * t;
-  }
-  return sqrt(s);
-}
-
-// SG flux of local edge j of element e, oriented from its first to its second
-// local vertex: mu sigma / L * (n_b B(phi_b - phi_a) - n_a B(phi_a - phi_b)).
+// SG flux of local edge (a, b) (a, b compile-time) with coefficient coef,
+// oriented from a to b: coef / L (n_b B(phi_b - phi_a) - n_a B(phi_a - phi_b)).
 // One Bernoulli evaluation per edge: with t = |x|, B(-t) = t + B(t) (both
 // terms positive, no cancellation).
-__device__ __forceinline__ double sg_flux(const double *__restrict__ xyz, int a, int b,
-                                          double coef, const double *__restrict__ phi,
-                                          const double *__restrict__ dens) {
-  const double L = edge_len(xyz, a, b);
-  const double dab = phi[b] - phi[a];
+__device__ __forceinline__ double sg_edge_flux(const double P[4][3], const double ph[4],
+                                               const double dn[4], int a, int b, double coef) {
+  double s2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double tt = P[a][d] - P[b][d];
+    s2 += tt * tt;
+  }
+  const double L = sqrt(s2);
+  const double dab = ph[b] - ph[a];
   const double t = fabs(dab);
   const double bt = bernoulli(t), bmt = t + bt;  // B(t), B(-t)
   const double bp = dab >= 0.0 ? bt : bmt;       // B(dab)
   const double bm = dab >= 0.0 ? bmt : bt;       // B(-dab)
-  return coef / L * (dens[b] * bp - dens[a] * bm);
-}
-
-// vertex id of local vertex q (q uniform, id in registers: selects, no
-// indexed register access)
-__device__ __forceinline__ int pick4(const int32_t *id, int q) {
-  return q == 0 ? id[0] : q == 1 ? id[1] : q == 2 ? id[2] : id[3];
+  return coef / L * (dn[b] * bp - dn[a] * bm);
 }
 
 // The residual in two passes.  k_dd_elem: one lane per element computes its
@@ -297,35 +293,48 @@ __global__ __launch_bounds__(128) void k_dd_elem(
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nT) return;
   const int4 ps = reinterpret_cast<const int4 *>(pos)[e];
-  const int32_t at[4] = {ps.x, ps.y, ps.z, ps.w};
   const int4 t = reinterpret_cast<const int4 *>(tets)[e];
   const int32_t id[4] = {t.x, t.y, t.z, t.w};
-  double G[4][3];
-  const double vol = p1_grads(xyz, id, G);
-  double gp[3] = {0, 0, 0};
-  for (int q = 0; q < 4; ++q)
-    for (int d = 0; d < 3; ++d) gp[d] += phi[id[q]] * G[q][d];
+  // every per-element and per-vertex operand is loaded once, up front
+  double P[4][3], ph[4], dn[4], sg[6];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) P[q][d] = xyz[3 * (size_t)id[q] + d];
+    ph[q] = phi[id[q]];
+    dn[q] = dens[id[q]];
+  }
+#pragma unroll
+  for (int j = 0; j < 6; ++j) sg[j] = sigma[6 * e + j];
   const double ee = eps[e], re = rho[e], me = mu[e];
-  // the 6 edges one at a time (a rolled loop keeps the register count, and
-  // with it the occupancy, of the latency-bound gathers low); each local
-  // vertex receives +-F_j in j order
+  // Poisson terms first, so the geometry is dead during the SG part (100
+  // VGPRs, 4 waves per SIMD, against 148 with the geometry kept live)
+  double cp[4];
+  {
+    double G[4][3];
+    const double vol = p1_geometry(P, G) / 6.0;
+    double gp[3] = {0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gp[k] += ph[q] * G[q][k];
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+      cp[l] = ee * vol * (G[l][0] * gp[0] + G[l][1] * gp[1] + G[l][2] * gp[2]) - re * vol / 4.0;
+  }
+  // each local vertex receives +-F_j of its edges in j order
   double cn[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 1
+#pragma unroll
   for (int j = 0; j < 6; ++j) {
-    const int qa = c_ev[j][0], qb = c_ev[j][1];
-    const double F = sg_flux(xyz, pick4(id, qa), pick4(id, qb), me * sigma[6 * e + j], phi, dens);
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      if (qa == l) cn[l] += F;
-      if (qb == l) cn[l] -= F;
-    }
+    const int a = kEA[j], b = kEB[j];
+    const double F = sg_edge_flux(P, ph, dn, a, b, me * sg[j]);
+    cn[a] += F;
+    cn[b] -= F;
   }
-#pragma unroll
-  for (int l = 0; l < 4; ++l) {
-    const double cp = ee * vol * (G[l][0] * gp[0] + G[l][1] * gp[1] + G[l][2] * gp[2]) -
-                      re * vol / 4.0;
-    contrib[at[l]] = make_double2(cp, cn[l]);
-  }
+  contrib[ps.x] = make_double2(cp[0], cn[0]);
+  contrib[ps.y] = make_double2(cp[1], cn[1]);
+  contrib[ps.z] = make_double2(cp[2], cn[2]);
+  contrib[ps.w] = make_double2(cp[3], cn[3]);
 }
 
 __global__ __launch_bounds__(256) void k_dd_gather(int64_t nV, const int32_t *__restrict__ vptr,
@@ -368,28 +377,45 @@ __global__ __launch_bounds__(256) void k_dd_gather(int64_t nV, const int32_t *__
   }
 }
 
-__global__ void k_dd_vjp(int64_t nT, const int32_t *__restrict__ tets,
-                         const double *__restrict__ xyz, const double *__restrict__ sigma,
-                         const double *__restrict__ phi, const double *__restrict__ dens,
-                         const double *__restrict__ rphi, const double *__restrict__ rn,
-                         double *__restrict__ g_eps, double *__restrict__ g_mu) {
+__global__ __launch_bounds__(128) void k_dd_vjp(
+    int64_t nT, const int32_t *__restrict__ tets, const double *__restrict__ xyz,
+    const double *__restrict__ sigma, const double *__restrict__ phi,
+    const double *__restrict__ dens, const double *__restrict__ rphi,
+    const double *__restrict__ rn, double *__restrict__ g_eps, double *__restrict__ g_mu) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nT) return;
   const int4 t = reinterpret_cast<const int4 *>(tets)[e];
   const int32_t id[4] = {t.x, t.y, t.z, t.w};
-  double G[4][3];
-  const double vol = p1_grads(xyz, id, G);
-  double gp[3] = {0, 0, 0};
-  for (int q = 0; q < 4; ++q)
-    for (int d = 0; d < 3; ++d) gp[d] += phi[id[q]] * G[q][d];
+  double P[4][3], ph[4], dn[4], rp[4], rv[4], sg[6];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) P[q][d] = xyz[3 * (size_t)id[q] + d];
+    ph[q] = phi[id[q]];
+    dn[q] = dens[id[q]];
+    rp[q] = rphi[id[q]];
+    rv[q] = rn[id[q]];
+  }
+#pragma unroll
+  for (int j = 0; j < 6; ++j) sg[j] = sigma[6 * e + j];
   double ge = 0.0;
-  for (int q = 0; q < 4; ++q)
-    ge += 2.0 * rphi[id[q]] * vol * (G[q][0] * gp[0] + G[q][1] * gp[1] + G[q][2] * gp[2]);
+  {
+    double G[4][3];
+    const double vol = p1_geometry(P, G) / 6.0;
+    double gp[3] = {0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gp[k] += ph[q] * G[q][k];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      ge += 2.0 * rp[q] * vol * (G[q][0] * gp[0] + G[q][1] * gp[1] + G[q][2] * gp[2]);
+  }
   double gm = 0.0;
+#pragma unroll
   for (int j = 0; j < 6; ++j) {
-    const int a = id[c_ev[j][0]], b = id[c_ev[j][1]];
-    const double f = sg_flux(xyz, a, b, sigma[6 * e + j], phi, dens);
-    gm += 2.0 * (rn[a] - rn[b]) * f;
+    const int a = kEA[j], b = kEB[j];
+    gm += 2.0 * (rv[a] - rv[b]) * sg_edge_flux(P, ph, dn, a, b, sg[j]);
   }
   g_eps[e] = ge;
   g_mu[e] = gm;
