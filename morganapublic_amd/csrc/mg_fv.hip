@@ -18,8 +18,6 @@ namespace mg {
 __constant__ int c_ev[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
 // the same edge table for compile-time indexing (local edge j = (kEA[j], kEB[j]))
 constexpr int kEA[6] = {0, 1, 2, 0, 1, 2}, kEB[6] = {1, 2, 0, 3, 3, 3};
-__constant__ int c_fv[4][3] = {{0, 2, 1}, {0, 1, 3}, {1, 2, 3}, {0, 3, 2}};  // faceToPoint
-__constant__ int c_fe[4][3] = {{2, 1, 0}, {0, 4, 3}, {1, 5, 4}, {3, 5, 2}};  // faceToEdge
 
 __device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {
   c[0] = a[1] * b[2] - a[2] * b[1];
@@ -29,59 +27,73 @@ __device__ __forceinline__ void cross3(const double *a, const double *b, double 
 
 // per element: aligned interface normal sum for each local edge, and the
 // sub-cell volume each local vertex receives
-__global__ void k_dual_elem(int64_t nT, const int32_t *__restrict__ tets,
+__global__ __launch_bounds__(128) void k_dual_elem(int64_t nT, const int32_t *__restrict__ tets,
                             const double *__restrict__ xyz, double *__restrict__ ne,
                             double *__restrict__ subvol) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nT) return;
   int32_t tv[4];
   double P[4][3];
+#pragma unroll
   for (int v = 0; v < 4; ++v) {
     tv[v] = tets[4 * e + v];
+#pragma unroll
     for (int d = 0; d < 3; ++d) P[v][d] = xyz[3 * tv[v] + d];
   }
-  double BA[3], F[4][3], Em[6][3], N[6][3], sv[4] = {0, 0, 0, 0};
+  double BA[3], sv[4] = {0, 0, 0, 0};
+#pragma unroll
   for (int d = 0; d < 3; ++d) BA[d] = (P[0][d] + P[1][d] + P[2][d] + P[3][d]) / 4.0;
-  for (int f = 0; f < 4; ++f)
-    for (int d = 0; d < 3; ++d)
-      F[f][d] = (P[c_fv[f][0]][d] + P[c_fv[f][1]][d] + P[c_fv[f][2]][d]) / 3.0;
-  for (int j = 0; j < 6; ++j)
+  // edge by edge, with its two faces (the faces of FV holding both ends);
+  // face centroids and edge midpoints are formed where used, so only the
+  // coordinates, the barycentre and the sub-volumes stay live (no scratch)
+  constexpr int FV[4][3] = {{0, 2, 1}, {0, 1, 3}, {1, 2, 3}, {0, 3, 2}};
+  constexpr int EF[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 3}, {1, 2}, {2, 3}};
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    double Em[3], N[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) Em[d] = (P[kEA[j]][d] + P[kEB[j]][d]) / 2.0;
+    // edge vector from the lower to the higher global vertex (a select, not a
+    // swap of indices: P stays indexed by constants)
+    const bool fwd = tv[kEA[j]] <= tv[kEB[j]];
+    double t[3];
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
-      Em[j][d] = (P[c_ev[j][0]][d] + P[c_ev[j][1]][d]) / 2.0;
-      N[j][d] = 0.0;
+      const double u = P[kEB[j]][d] - P[kEA[j]][d];
+      t[d] = fwd ? u : -u;
     }
-  for (int f = 0; f < 4; ++f)
-    for (int k = 0; k < 3; ++k) {
-      const int j = c_fe[f][k];
-      double a[3], b[3], n[3], t[3];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int f = EF[j][q];
+      double F[3], a[3], b[3], n[3];
+#pragma unroll
       for (int d = 0; d < 3; ++d) {
-        a[d] = F[f][d] - Em[j][d];
-        b[d] = BA[d] - Em[j][d];
+        F[d] = (P[FV[f][0]][d] + P[FV[f][1]][d] + P[FV[f][2]][d]) / 3.0;
+        a[d] = F[d] - Em[d];
+        b[d] = BA[d] - Em[d];
       }
       cross3(a, b, n);
-      int lo = c_ev[j][0], hi = c_ev[j][1];
-      if (tv[lo] > tv[hi]) {
-        int s = lo;
-        lo = hi;
-        hi = s;
-      }
-      for (int d = 0; d < 3; ++d) t[d] = P[hi][d] - P[lo][d];
       const double sgn = (n[0] * t[0] + n[1] * t[1] + n[2] * t[2]) >= 0.0 ? 1.0 : -1.0;
-      for (int d = 0; d < 3; ++d) N[j][d] += sgn * n[d];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) N[d] += sgn * n[d];
+      // the two sub-volumes (edge midpoint, face centroid, barycentre, end vertex)
+      double m[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        a[d] = Em[d] - BA[d];
+        b[d] = F[d] - BA[d];
+      }
+      cross3(a, b, m);
+#pragma unroll
       for (int end = 0; end < 2; ++end) {
-        const int v = c_ev[j][end];
-        double m[3], cc[3];
-        for (int d = 0; d < 3; ++d) {
-          a[d] = Em[j][d] - BA[d];
-          b[d] = F[f][d] - BA[d];
-          cc[d] = P[v][d] - BA[d];
-        }
-        cross3(a, b, m);
-        sv[v] += fabs(m[0] * cc[0] + m[1] * cc[1] + m[2] * cc[2]) / 6.0;
+        const int v = end == 0 ? kEA[j] : kEB[j];
+        sv[v] += fabs(m[0] * (P[v][0] - BA[0]) + m[1] * (P[v][1] - BA[1]) +
+                      m[2] * (P[v][2] - BA[2])) / 6.0;
       }
     }
-  for (int j = 0; j < 6; ++j)
-    for (int d = 0; d < 3; ++d) ne[18 * e + 3 * j + d] = N[j][d];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ne[18 * e + 3 * j + d] = N[d];
+  }
   for (int v = 0; v < 4; ++v) subvol[4 * e + v] = sv[v];
 }
 
