@@ -194,6 +194,18 @@ static EdgePack pack_edges() {
   return p;
 }
 
+// value k of six held in registers (by value: a select over loads from a
+// local array can be folded back into a runtime index, which puts the array
+// in scratch memory)
+template <typename T>
+__device__ __forceinline__ T pick6v(T s0, T s1, T s2, T s3, T s4, T s5, uint32_t k) {
+  const T a = (k & 1u) ? s1 : s0;
+  const T b = (k & 1u) ? s3 : s2;
+  const T c = (k & 1u) ? s5 : s4;
+  const T ab = (k & 2u) ? b : a;
+  return (k & 4u) ? c : ab;
+}
+
 template <typename T>
 __device__ __forceinline__ T pick6(const T *s, uint32_t k) {
   const T a = (k & 1u) ? s[1] : s[0];
@@ -416,20 +428,51 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
       T e[6];
       T md;
     };
-    auto load_vals = [&](Slot &A) {
-      const int l = A.r.el & 15;
-      uint32_t w;
+    // relabelled edge j of incidence basis l is original edge (w >> 3j) & 7
+    auto edge_word = [&](int l) {
       if (LROW == 0) {
         const uint32_t a = (l & 1) ? ep.w[1] : ep.w[0], b = (l & 1) ? ep.w[3] : ep.w[2];
-        w = (l & 2) ? b : a;
-      } else {
-        w = pick6(ep.w + 4, (uint32_t)(l - 4));
+        return (l & 2) ? b : a;
       }
+      return pick6(ep.w + 4, (uint32_t)(l - 4));
+    };
+    auto load_vals = [&](Slot &A) {
+      const uint32_t w = edge_word(A.r.el & 15);
       const T *g = geo + (size_t)(A.r.el >> 4) * GS;
+      // FP32 records and FP64 edge rows: the whole record in 16-byte loads,
+      // the needed values selected in registers.  A gather is paced by the
+      // cache lines it touches (one per lane and load), not by its bytes:
+      // measured -11% FP32 edge rows (2 loads instead of 5), -7% FP32
+      // vertex rows (2 instead of 3), -3% FP64 edge rows (3 instead of 5);
+      // FP64 vertex rows (3 either way) are slower with the wider registers.
+      constexpr bool kWide = sizeof(T) == 4 || LROW == 4;
+      if constexpr (kWide) {
+        T f[8];
+        if constexpr (sizeof(T) == 8) {
 #pragma unroll
-      for (int j = 0; j < 6; ++j)
-        A.e[j] = edge_needed<LROW, EXACT>(j) ? g[(w >> (3 * j)) & 7u] : (T)0;
-      A.md = MASS ? g[6] : (T)0;
+          for (int k = 0; k < GS / 2; ++k) {
+            const double2 q = reinterpret_cast<const double2 *>(g)[k];
+            f[2 * k] = q.x;
+            f[2 * k + 1] = q.y;
+          }
+        } else {
+          const float4 q0 = reinterpret_cast<const float4 *>(g)[0];
+          const float4 q1 = reinterpret_cast<const float4 *>(g)[1];
+          f[0] = q0.x, f[1] = q0.y, f[2] = q0.z, f[3] = q0.w;
+          f[4] = q1.x, f[5] = q1.y, f[6] = q1.z, f[7] = q1.w;
+        }
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+          A.e[j] = edge_needed<LROW, EXACT>(j)
+                       ? pick6v(f[0], f[1], f[2], f[3], f[4], f[5], (w >> (3 * j)) & 7u)
+                       : (T)0;
+        A.md = MASS ? f[6] : (T)0;
+      } else {
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+          A.e[j] = edge_needed<LROW, EXACT>(j) ? g[(w >> (3 * j)) & 7u] : (T)0;
+        A.md = MASS ? g[6] : (T)0;
+      }
     };
     // add incidence j's row of the element matrix (slot A) to the row's entries
     auto add = [&](const Slot &A, int j) {
