@@ -364,8 +364,9 @@ __device__ __forceinline__ void row_values(const T e[6], T md, T v[NL]) {
 // incidences in turn (lane sub takes sub, sub + LPR, ...) and add their rows
 // of the element matrices in LPR ordered phases, so every entry sums its
 // contributions in incidence order, independent of ROWS and LPR.  An
-// incidence reads its record, then only the relabelled edge values its row
-// needs (gathered by address from the element record, no selects).
+// incidence reads its record, then its element's edge values: an FP64
+// vertex row gathers the 3 it needs by address; edge rows and FP32 rows load
+// the whole element record in 16-byte loads and select in registers.
 // An FP64 vertex row's incidences (about 24) are software-pipelined: while
 // incidence j is added, the edge values of j+1 and the record of j+3 are in
 // flight; edge rows (about 5 incidences) and FP32 rows load theirs in chunks
