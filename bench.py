@@ -137,6 +137,20 @@ def bench_config5(ctx, timer, n=128):
     uobs = (0.01 * (2 * W.stream_uniform(0, W.STREAM_UOBS, fixed.size) - 1))[fixed == 0]
     du, dl, dg = timer.zeros(8 * nint), timer.zeros(8 * nint), timer.zeros(8 * m.nT)
     out = {"tets": m.nT, "dofs_interior": nint, "nnz_interior": nnz_int}
+    # the P1 assembly alone (interior system; algorithmic bytes with nnz_int)
+    kw = dict(stiff_coeff=L.COEFF_P0, stiff_rule=1)
+    b_asm = m.nT * (4 * 4 + 8) + 24 * m.nV + 8 * nnz_int
+    for prec, name in ((L.PREC_F64, "f64"), (L.PREC_F32, "f32")):
+        ctx.assemble(L.SYS_INTERIOR, L.TERM_STIFF, prec=prec, **kw)
+        timer.sync()
+        timer.start()
+        for _ in range(10):
+            ctx.assemble(L.SYS_INTERIOR, L.TERM_STIFF, prec=prec, **kw)
+        ms = timer.stop_ms() / 10
+        out["assembly_%s_us" % name] = round(1e3 * ms, 1)
+        if prec == L.PREC_F64:
+            out["assembly_f64_roofline_frac"] = round(b_asm / (ms * 1e-3) / 8e12, 4)
+        log("cfg5 P1 assembly %s: %.1f us (%.0f M elem/s)" % (name, 1e3 * ms, m.nT / ms / 1e3))
     grads = {}
     for name, mode in (("f64", L.SOLVE_F64), ("f32_refine", L.SOLVE_F32_REFINE),
                        ("f32_reliable", L.SOLVE_F32_RELIABLE)):

@@ -1,10 +1,11 @@
 // Numeric assembly of P1/P2 tetrahedral operators into a fixed CSR pattern,
 // and the adjoint element kernel for d(loss)/d(per-element coefficient).
 //
-// An assembly is three launches.  k_elem_factors: one lane per element
-// computes, in FP64 from the vertex coordinates, the 6 edge factors
-// S_ab = c vol grad(lambda_a).grad(lambda_b) (and c_mass det with a mass
-// term), stored per element.  k_asm_rows, once over the vertex-dof rows and
+// A P2 assembly is three launches, a P1 assembly two (FP32) or one (FP64).
+// k_elem_factors: one lane per element computes, in FP64 from the vertex
+// coordinates, the 6 edge factors S_ab = c vol grad(lambda_a).grad(lambda_b)
+// (and c_mass det with a mass term), stored per element (FP64 P1 rows form
+// them in place instead).  k_asm_rows, once over the vertex-dof rows and
 // once over the P2 edge-dof rows: a workgroup owns consecutive CSR rows and
 // stages their value range in LDS; LPR lanes per row walk the row's
 // (element, local basis l) incidence records, whose element is relabelled by
@@ -122,6 +123,40 @@ constexpr int geo_stride() {
   return (sizeof(T) == 8 && !MASS) ? 6 : 8;
 }
 
+// The 6 edge factors (kEdgeV order) and c_mass det of one element from its
+// vertex coordinates: the arithmetic of k_elem_factors, shared with the P1
+// row kernels that form the factors in place.
+template <typename T, bool MASS>
+__device__ __forceinline__ void edge_factors(const double P[4][3], double ks, double km,
+                                             T out[8]) {
+  double d[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) d[k][r] = P[k + 1][r] - P[0][r];
+  double n[4][3];
+  n[1][0] = d[1][1] * d[2][2] - d[1][2] * d[2][1];
+  n[1][1] = d[1][2] * d[2][0] - d[1][0] * d[2][2];
+  n[1][2] = d[1][0] * d[2][1] - d[1][1] * d[2][0];
+  n[2][0] = d[2][1] * d[0][2] - d[2][2] * d[0][1];
+  n[2][1] = d[2][2] * d[0][0] - d[2][0] * d[0][2];
+  n[2][2] = d[2][0] * d[0][1] - d[2][1] * d[0][0];
+  n[3][0] = d[0][1] * d[1][2] - d[0][2] * d[1][1];
+  n[3][1] = d[0][2] * d[1][0] - d[0][0] * d[1][2];
+  n[3][2] = d[0][0] * d[1][1] - d[0][1] * d[1][0];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) n[0][r] = -(n[1][r] + n[2][r] + n[3][r]);
+  const double det = d[0][0] * n[1][0] + d[0][1] * n[1][1] + d[0][2] * n[1][2];
+  const double sc = ks / (6.0 * det);
+  constexpr int EA[6] = {0, 1, 2, 0, 1, 2}, EB[6] = {1, 2, 0, 3, 3, 3};  // kEdgeV
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+    out[j] = (T)(sc * (n[EA[j]][0] * n[EB[j]][0] + n[EA[j]][1] * n[EB[j]][1] +
+                       n[EA[j]][2] * n[EB[j]][2]));
+  out[6] = MASS ? (T)(km * det) : (T)0;
+  out[7] = (T)0;
+}
+
 template <typename T, bool MASS>
 __global__ __launch_bounds__(256) void k_elem_factors(
     int64_t nT, const int32_t *__restrict__ tets, const double *__restrict__ xyz,
@@ -144,35 +179,11 @@ __global__ __launch_bounds__(256) void k_elem_factors(
     P[v][1] = q.y;
     P[v][2] = xyz[3 * (size_t)id[v] + 2];
   }
-  double d[3][3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-#pragma unroll
-    for (int r = 0; r < 3; ++r) d[k][r] = P[k + 1][r] - P[0][r];
-  double n[4][3];
-  n[1][0] = d[1][1] * d[2][2] - d[1][2] * d[2][1];
-  n[1][1] = d[1][2] * d[2][0] - d[1][0] * d[2][2];
-  n[1][2] = d[1][0] * d[2][1] - d[1][1] * d[2][0];
-  n[2][0] = d[2][1] * d[0][2] - d[2][2] * d[0][1];
-  n[2][1] = d[2][2] * d[0][0] - d[2][0] * d[0][2];
-  n[2][2] = d[2][0] * d[0][1] - d[2][1] * d[0][0];
-  n[3][0] = d[0][1] * d[1][2] - d[0][2] * d[1][1];
-  n[3][1] = d[0][2] * d[1][0] - d[0][0] * d[1][2];
-  n[3][2] = d[0][0] * d[1][1] - d[0][1] * d[1][0];
-#pragma unroll
-  for (int r = 0; r < 3; ++r) n[0][r] = -(n[1][r] + n[2][r] + n[3][r]);
-  const double det = d[0][0] * n[1][0] + d[0][1] * n[1][1] + d[0][2] * n[1][2];
   const double ks = ks_field ? ks_field[ec] : ks_const;
-  const double sc = ks / (6.0 * det);
-  constexpr int EA[6] = {0, 1, 2, 0, 1, 2}, EB[6] = {1, 2, 0, 3, 3, 3};  // kEdgeV
+  const double km = MASS ? (km_field ? km_field[ec] : km_const) : 0.0;
   constexpr int N = geo_stride<T, MASS>();
   T out[8];
-#pragma unroll
-  for (int j = 0; j < 6; ++j)
-    out[j] = (T)(sc * (n[EA[j]][0] * n[EB[j]][0] + n[EA[j]][1] * n[EB[j]][1] +
-                       n[EA[j]][2] * n[EB[j]][2]));
-  out[6] = MASS ? (T)((km_field ? km_field[ec] : km_const) * det) : (T)0;
-  out[7] = (T)0;
+  edge_factors<T, MASS>(P, ks, km, out);
   // the block's records are staged in LDS and leave as one contiguous run of
   // 16-byte stores (per-lane records 48 bytes apart: measured -25% for this
   // kernel, -2% for the FP64 assembly)
@@ -191,6 +202,14 @@ __global__ __launch_bounds__(256) void k_elem_factors(
     for (int i = threadIdx.x; i < nb * N / 4; i += 256) dst[i] = src[i];
   }
 }
+
+// What the P1 row kernels need to form element factors in place.
+struct ElemSrc {
+  const int32_t *tets;
+  const double *xyz;
+  const double *ksf, *kmf;
+  double ksc, kmc;
+};
 
 // The element of an incidence is relabelled by the even vertex permutation
 // that moves the row's basis to local index 0 (vertex) or 4 (edge (0,1)); the
@@ -393,7 +412,7 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
     int32_t row_begin, int32_t row_end, int32_t dummy, EdgePack ep,
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ inc_ptr,
     const int32_t *__restrict__ diagpos, const Rec *__restrict__ rec, const T *__restrict__ geo,
-    T *__restrict__ vals) {
+    ElemSrc es, T *__restrict__ vals) {
   extern __shared__ __align__(16) char lds_raw[];
   T *acc = reinterpret_cast<T *>(lds_raw);
   constexpr int NT = ROWS * LPR;
@@ -409,7 +428,12 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
   // together after it (one round trip; otherwise the row offset is loaded
   // only after the loop guard has waited for the incidence range).
   // Measured at 64^3 P2: FP64 vertex rows are fastest pipelined.
-  constexpr bool kChunked = LROW == 4 || sizeof(T) == 4;
+  // FP64 P1 rows form their element factors in place (no factor kernel, no
+  // factor array): measured 706 -> 593 us per 128^3 P1 assembly.  FP32 P1
+  // rows and all P2 rows read the factor records (in place: FP32 P1 +19%,
+  // P2 FP64 the same, P2 FP32 +13%).
+  constexpr bool kInl = NL == 4 && sizeof(T) == 8;
+  constexpr bool kChunked = kInl || LROW == 4 || sizeof(T) == 4;
   const int r = r0 + lane / LPR;
   const int rr = kChunked ? min(r, r1 - 1) : r;
   // compact records leave out the row's own column: its position is the
@@ -529,7 +553,56 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
         if (LPR > 1) asm volatile("" ::: "memory");
       }
     };
-    if constexpr (kChunked) {
+    if constexpr (kInl) {
+      // A lane takes 3 incidences at once (2: 626 us, 4: 764 us at 128^3):
+      // their records, then vertex ids and coefficients, then the vertex
+      // coordinates, and forms each element's factors with the arithmetic of
+      // k_elem_factors (the values are bit-identical to the factor path).
+      constexpr int KC = 3;
+      const int4 *__restrict__ tets4 = reinterpret_cast<const int4 *>(es.tets);
+      typedef double dbl2_a8 __attribute__((ext_vector_type(2), aligned(8)));
+      for (; i < iend; i += KC * LPR) {
+        RV R[KC];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) R[k] = RV::load(rec, at(i + k * LPR), self);
+        int4 t[KC];
+        double ks[KC], km[KC];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+          const int el = R[k].el >> 4;
+          t[k] = tets4[el];
+          ks[k] = es.ksf ? es.ksf[el] : es.ksc;
+          km[k] = MASS ? (es.kmf ? es.kmf[el] : es.kmc) : 0.0;
+        }
+        double P[KC][4][3];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+          const int id[4] = {t[k].x, t[k].y, t[k].z, t[k].w};
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const dbl2_a8 q = *reinterpret_cast<const dbl2_a8 *>(es.xyz + 3 * (size_t)id[v]);
+            P[k][v][0] = q.x;
+            P[k][v][1] = q.y;
+            P[k][v][2] = es.xyz[3 * (size_t)id[v] + 2];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+          T f[8];
+          edge_factors<T, MASS>(P[k], ks[k], km[k], f);
+          const uint32_t w = edge_word(R[k].el & 15);
+          Slot A;
+          A.r = R[k];
+#pragma unroll
+          for (int j = 0; j < 6; ++j)
+            A.e[j] = edge_needed<LROW, EXACT>(j)
+                         ? pick6v(f[0], f[1], f[2], f[3], f[4], f[5], (w >> (3 * j)) & 7u)
+                         : (T)0;
+          A.md = MASS ? f[6] : (T)0;
+          add(A, i + k * LPR);
+        }
+      }
+    } else if constexpr (kChunked) {
       // A lane takes up to KC incidences at once, records first, then all
       // their edge values, so a P2 edge row (about 5 incidences) or a vertex
       // row (about 24, 6 per lane) needs two load round trips, with nothing
@@ -588,16 +661,18 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
   const bool exact = (!stiff || rule_exact(c->order, MG_TERM_STIFF, op->stiff_rule)) &&
                      (!mass || rule_exact(c->order, MG_TERM_MASS, op->mass_rule));
   static const EdgePack ep = pack_edges();
+  const ElemSrc es{c->tets, c->xyz, ksf, kmf, ksc, kmc};
+  const bool inl = NL == 4 && sizeof(T) == 8;  // as kInl in k_asm_rows
   // element records
   const int stride = (sizeof(T) == 8 && !mass) ? 6 : 8;
   const size_t gbytes = sizeof(T) * stride * (size_t)c->nT;
-  if (c->geom_bytes < gbytes) {
+  if (!inl && c->geom_bytes < gbytes) {
     dfree(c, c->geom);
     c->geom = dalloc<char>(c, gbytes);
     c->geom_bytes = gbytes;
   }
   T *geo = (T *)c->geom;
-  if (c->nT > 0) {
+  if (c->nT > 0 && !inl) {
     if (mass)
       k_elem_factors<T, true><<<blocks_for(c->nT, 256), 256, 0, c->stream>>>(
           c->nT, c->tets, c->xyz, ksc, ksf, kmc, kmf, geo);
@@ -616,13 +691,13 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
                    decltype(rw)::value, decltype(lp)::value, Rec>
             <<<nblk, decltype(rw)::value * decltype(lp)::value, lds, c->stream>>>(
                 sg.row_begin, sg.row_end, sg.lds_slots, ep, S.rowptr, S.inc_ptr, S.diagpos, rec, geo,
-                vals);
+                es, vals);
       else
         k_asm_rows<NL, decltype(lr)::value, T, decltype(ms)::value, decltype(ex)::value, false,
                    decltype(rw)::value, decltype(lp)::value, Rec>
           <<<nblk, decltype(rw)::value * decltype(lp)::value, lds, c->stream>>>(
               sg.row_begin, sg.row_end, sg.lds_slots, ep, S.rowptr, S.inc_ptr, S.diagpos, rec, geo,
-                vals);
+                es, vals);
     };
     // vertex-dof rows: 64 rows per workgroup, 1/2/4 lanes per row; P2 edge-dof
     // rows: 64/128/256 rows per workgroup, 1/2 lanes per row
