@@ -445,7 +445,9 @@ static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvr
   // launch shapes (MGFE_VERTEX_LPR 1/2/4, MGFE_EDGE_ROWS 64/128/256,
   // MGFE_EDGE_LPR 1/2) only change speed, never the result; defaults from a
   // sweep at 64^3 P2 (edge rows: 128 x 2 lanes 0.283-0.288 ms per FP64
-  // assembly, 256 x 2 0.292-0.305, 64 x 2 0.292; 4 lanes per edge row 0.39+)
+  // assembly, 256 x 2 0.292-0.305, 64 x 2 0.292; 4 lanes per edge row 0.39+;
+  // with whole-record loads, one lane per row: 128 x 1 0.284-0.287, 256 x 1
+  // 0.278-0.280, 64 x 1 0.292-0.294 against 0.273-0.277 for 128 x 2)
   const int vl = env_int("MGFE_VERTEX_LPR", 4), er = env_int("MGFE_EDGE_ROWS", 128),
             el = env_int("MGFE_EDGE_LPR", 2);
   add(0, nvrows, 0, 64, pick(vl, 1, 2, 4) ? vl : 4);
