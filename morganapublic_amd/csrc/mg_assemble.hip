@@ -399,8 +399,8 @@ __device__ __forceinline__ void row_values(const T e[6], T md, T v[NL]) {
 }
 
 // ROWS rows per workgroup, LPR lanes per row, the block's CSR entries staged
-// in LDS (plus a 256-slot dummy region that takes the contributions of
-// padding steps and eliminated columns).  The lanes of a row take its
+// in LDS (plus a 256-slot dummy region that takes the contributions to
+// eliminated columns).  The lanes of a row take its
 // incidences in turn (lane sub takes sub, sub + LPR, ...) and add their rows
 // of the element matrices in LPR ordered phases, so every entry sums its
 // contributions in incidence order, independent of ROWS and LPR.  An
@@ -464,9 +464,10 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
     }
     // Pipelined form: three slots rotate; unrolled by three so every load
     // lands in the slot that consumes it and no loaded register is copied.
-    // Both forms are branch-free: indices past the row's last incidence are
-    // clamped (loads stay in bounds) and their contribution goes to the
-    // dummy region.
+    // In both forms, indices past the row's last incidence are clamped (loads
+    // stay in bounds) and their LDS update is masked off (measured -1.5%
+    // against sending it to the dummy region, where the padding lanes of
+    // all rows hit the same slots).
     const int last = iend > i - sub ? iend - 1 : 0;  // a row without incidences reads record 0
     auto at = [&](int j) { return j < iend ? j : last; };
     struct Slot {
@@ -545,12 +546,13 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
       // ds_add_f32 several times slower than the plain form).  In a phase
       // every active lane owns a different row, so the plain form has no
       // conflicts (the dummy region excepted, whose values are discarded).
+      // Padding steps (j past the row's last incidence) add nothing.
       // The lanes of a row are in one wavefront, whose LDS operations
       // execute in issue order; the compiler barrier keeps the phases in
       // that order.
 #pragma unroll
       for (int ph = 0; ph < LPR; ++ph) {
-        if (LPR == 1 || sub == ph) {
+        if ((LPR == 1 || sub == ph) && valid) {
           if constexpr (sizeof(T) == 8 && LROW == 0) {
 #pragma unroll
             for (int m = 0; m < NL; ++m) unsafeAtomicAdd(&acc[idx[m]], v[m]);
