@@ -717,9 +717,10 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
     // rows: 64/128/256 rows per workgroup, 1/2 lanes per row
     auto shape = [&](auto lr, auto ms, auto ex) {
       if constexpr (decltype(lr)::value == 0) {
-        if (sg.lpr == 4)
+        const int lpr = sizeof(T) == 4 ? sg.lpr32 : sg.lpr;
+        if (lpr == 4)
           go(lr, ms, ex, IC<64>{}, IC<4>{});
-        else if (sg.lpr == 2)
+        else if (lpr == 2)
           go(lr, ms, ex, IC<64>{}, IC<2>{});
         else
           go(lr, ms, ex, IC<64>{}, IC<1>{});

@@ -41,6 +41,7 @@ struct alignas(4) RecP2c {
 struct RowSegment {
   int32_t row_begin, row_end, lds_slots, lrow;
   int32_t rows, lpr;  // rows per workgroup, lanes per row
+  int32_t lpr32;      // lanes per row of FP32 assemblies
 };
 
 constexpr int kSpmvRows = 64;  // rows per SpMV block (128 for short rows)

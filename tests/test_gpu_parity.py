@@ -164,6 +164,7 @@ def test_assembly_launch_shapes_bit_identical(ctx, monkeypatch, order, rs, prec,
     out = []
     for vl, er, el, r16 in SHAPES:
         monkeypatch.setenv("MGFE_VERTEX_LPR", vl)
+        monkeypatch.setenv("MGFE_VERTEX_LPR32", vl)
         monkeypatch.setenv("MGFE_EDGE_ROWS", er)
         monkeypatch.setenv("MGFE_EDGE_LPR", el)
         monkeypatch.setenv("MGFE_REC16", r16)
