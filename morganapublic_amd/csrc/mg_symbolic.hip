@@ -426,7 +426,7 @@ static int env_int(const char *name, int dflt) {
   return v ? atoi(v) : dflt;
 }
 
-static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvrows) {
+static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvrows, int nl) {
   S.segs.clear();
   auto add = [&](int64_t a, int64_t b, int lrow, int rows, int lpr, int lpr32) {
     if (b <= a) return;
@@ -448,11 +448,13 @@ static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvr
   // assembly, 256 x 2 0.292-0.305, 64 x 2 0.292; 4 lanes per edge row 0.39+;
   // with whole-record loads, one lane per row: 128 x 1 0.284-0.287, 256 x 1
   // 0.278-0.280, 64 x 1 0.292-0.294 against 0.273-0.277 for 128 x 2)
-  // FP32 vertex rows: 2 lanes per row (64 x 2: 166.9 us per 64^3 P2 FP32
-  // assembly, 64 x 4: 171.4-172.3, 64 x 1: 189; FP64 64 x 2 is 3% slower)
-  const int vl = env_int("MGFE_VERTEX_LPR", 4), vl32 = env_int("MGFE_VERTEX_LPR32", 2),
+  // FP32 P2 vertex rows: 2 lanes per row (64 x 2: 166.9 us per 64^3 P2 FP32
+  // assembly, 64 x 4: 171.4-172.3, 64 x 1: 189; FP64 64 x 2 is 3% slower);
+  // FP32 P1 vertex rows: 4 (128^3 P1: 404 us against 473 with 2)
+  const int vl = env_int("MGFE_VERTEX_LPR", 4),
+            vl32 = env_int("MGFE_VERTEX_LPR32", nl == 10 ? 2 : 4),
             er = env_int("MGFE_EDGE_ROWS", 128), el = env_int("MGFE_EDGE_LPR", 2);
-  add(0, nvrows, 0, 64, pick(vl, 1, 2, 4) ? vl : 4, pick(vl32, 1, 2, 4) ? vl32 : 2);
+  add(0, nvrows, 0, 64, pick(vl, 1, 2, 4) ? vl : 4, pick(vl32, 1, 2, 4) ? vl32 : (nl == 10 ? 2 : 4));
   const int erows = pick(er, 64, 128, 256) ? er : 128;
   const int elpr = (el == 1 || el == 2) ? el : 2;
   add(nvrows, S.nrows, 4, erows, elpr, elpr);
@@ -556,7 +558,7 @@ void build_pattern(Ctx *c, int sys) {
   MG_CHECK(hb == 0, MG_ERR_SIZE, "pattern: a row has more than 254 entries (uint8 positions)");
   const int64_t nvrows =
       nl == 4 ? nrows : (sys == MG_SYS_FULL ? c->nV : c->nfree_vertex);
-  make_segments(S, rp, nvrows);
+  make_segments(S, rp, nvrows, nl);
   {
     // SpMV row blocks
     std::vector<int32_t> sb;
