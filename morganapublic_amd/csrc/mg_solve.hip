@@ -373,7 +373,7 @@ __global__ __launch_bounds__(kThreads) void k_update(int n, T *__restrict__ x, T
         rv[j0 + u * kThreads] = R[u];
       }
     }
-    if (blockIdx.x == 0 && threadIdx.x < n - nv * VW) one(nv * VW + threadIdx.x);
+    if (blockIdx.x == 0 && (int)threadIdx.x < n - nv * VW) one(nv * VW + threadIdx.x);
   } else {
     for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) one(i);
   }
@@ -452,7 +452,7 @@ __global__ __launch_bounds__(kThreads) void k_pupdate(int n, const T *__restrict
         pv[j0 + u * kThreads] = P[u];
       }
     }
-    if (blockIdx.x == 0 && threadIdx.x < n - nv * VW) {
+    if (blockIdx.x == 0 && (int)threadIdx.x < n - nv * VW) {
       const int i = nv * VW + threadIdx.x;
       p[i] = dinv[i] * r[i] + beta * p[i];
     }
