@@ -17,6 +17,16 @@
  *                 per-element split sums to edgeSurf
  *   SG residual   Bernoulli identities, linear-profile closed form
  *   gradients     central finite differences of the oracle's own loss
+ *   load vector   sum F = |cube|; P1 vol/4, P2 -vol/20 (vertex) and vol/5
+ *                 (edge) closed forms; F = M f with the independently
+ *                 assembled mass matrix; linearity in f
+ *
+ * Parity unpinned: no function of this file.  Left open are parts of the
+ * reference outside it: Belos Block-CG's own convergence scaling and Ifpack's
+ * Jacobi defaults (Trilinos is not vendored; the criterion is the explicit
+ * FP64 residual instead), edge gids under pMapItemShare (the pMapItem
+ * lexicographic rule is restated), and Epetra's summation order in
+ * GlobalAssemble (below the 1e-12 tolerance).
  */
 #include "oracle.h"
 
