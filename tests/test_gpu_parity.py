@@ -135,6 +135,34 @@ def test_assembly_fp32(ctx, order, rs):
     assert err.max() <= 2e-6, err.max()
 
 
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("prec", [L.PREC_F64, L.PREC_F32])
+@pytest.mark.parametrize("sysid", [L.SYS_FULL, L.SYS_INTERIOR])
+@pytest.mark.parametrize("fields", [True, False])
+def test_assembly_coefficients(ctx, order, prec, sysid, fields):
+    """K + M with P0 fields on both terms, or with non-unit constants."""
+    m = W.kuhn_cube(4, jitter=0.2, seed=8)
+    if fields:
+        ks = np.exp(W.stream_normal(0, W.STREAM_K, m.nT))
+        km = 0.5 + W.stream_uniform(0, W.STREAM_RHO, m.nT)
+        kw = dict(stiff_coeff=L.COEFF_P0, mass_coeff=L.COEFF_P0)
+    else:
+        ks, km = np.full(m.nT, 2.5), np.full(m.nT, 0.75)
+        kw = dict(stiff_const=2.5, mass_const=0.75)
+    setup(ctx, m, order)
+    if fields:
+        ctx.field_set(L.FIELD_STIFF, ks)
+        ctx.field_set(L.FIELD_MASS, km)
+    ctx.assemble(sysid, L.TERM_STIFF | L.TERM_MASS, stiff_rule=4, mass_rule=15, prec=prec, **kw)
+    got = ctx.values(sysid, prec=prec).astype(np.float64)
+    rp, ci, va = O.assemble(order, m, O.OP_STIFF | O.OP_MASS, c_stiff=ks, c_mass=km,
+                            rule_stiff=4, rule_mass=15)
+    if sysid == L.SYS_INTERIOR:
+        rp, ci, va = O.restrict(rp, ci, va, O.dirichlet_mask(order, m, ALL_FACES))
+    err = np.abs(got - va) / row_scale(rp, va)
+    assert err.max() <= (1e-12 if prec == L.PREC_F64 else 2e-6), err.max()
+
+
 def test_assembly_deterministic(ctx):
     m = W.kuhn_cube(8, jitter=0.2, seed=3)
     setup(ctx, m, 2)
