@@ -128,8 +128,9 @@ struct StreamBlk {
   T xr;      // the row operand of the epilogue at this lane's row
 };
 
-// branch-free: every lane issues its PER (column, value) loads; lanes past
-// the block's end re-read entry s0 (and later store into the spare LDS slot)
+// branch-free: every lane issues its (column, value) loads for entries
+// PER*lane .. PER*lane + PER - 1; lanes past the block's end re-read entry s0,
+// and entries past it go to the spare LDS slot
 template <typename T, bool C16, int RL>
 __device__ __forceinline__ void load_blk(StreamBlk<T> &B, int b, const BlkDesc *__restrict__ desc,
                                          const int32_t *__restrict__ rowptr, const Cols cs,
@@ -153,12 +154,42 @@ __device__ __forceinline__ void load_blk(StreamBlk<T> &B, int b, const BlkDesc *
   const int row = min(B.r0 + (int)(threadIdx.x / RL), B.r1);  // == r1: empty range
   B.ra = rowptr[row] - B.s0;
   B.re = rowptr[min(row + 1, B.r1)] - B.s0;
-#pragma unroll
-  for (int u = 0; u < StreamBlk<T>::PER; ++u) {
-    const int k = threadIdx.x + u * kThreads;
-    const int kc = B.s0 + (k < B.ns ? k : 0);
-    B.col[u] = C16 ? (int)cs.ofs[kc] : cs.idx[kc];
-    B.v[u] = vals[kc];
+  // a lane's PER entries are consecutive: the columns in one load, the
+  // values in one or two 16-byte loads (the arrays carry 8 entries of
+  // padding, so the last block's over-read stays in bounds).  Measured
+  // against PER loads strided by the workgroup size: -2.6% per FP64 PCG
+  // iteration, -1% per FP32 one.
+  static_assert(StreamBlk<T>::PER == 4, "four entries per lane");
+  const int k0 = threadIdx.x * 4;
+  const int kb = B.s0 + (k0 < B.ns ? k0 : 0);
+  // Entries past the block's end (up to 3) belong to the next block or to the
+  // zeroed padding; their products go to the spare LDS slot, and their
+  // gather indices stay inside x without a clamp: int32 entries are columns
+  // of the matrix (or 0), and the pattern build keeps every cluster base
+  // <= max(n - kColSpan, 0), so any 16-bit entry decodes inside [0, n)
+  // against any block's table.
+  if constexpr (C16) {
+    typedef uint32_t u3a4 __attribute__((ext_vector_type(3), aligned(4)));
+    const u3a4 w = *reinterpret_cast<const u3a4 *>(cs.ofs + (kb & ~1));
+    const bool odd = kb & 1;
+    B.col[0] = (int)(odd ? (w.x >> 16) : (w.x & 0xFFFFu));
+    B.col[1] = (int)(odd ? (w.y & 0xFFFFu) : (w.x >> 16));
+    B.col[2] = (int)(odd ? (w.y >> 16) : (w.y & 0xFFFFu));
+    B.col[3] = (int)(odd ? (w.z & 0xFFFFu) : (w.y >> 16));
+  } else {
+    typedef int i4a4 __attribute__((ext_vector_type(4), aligned(4)));
+    const i4a4 q = *reinterpret_cast<const i4a4 *>(cs.idx + kb);
+    B.col[0] = q.x, B.col[1] = q.y, B.col[2] = q.z, B.col[3] = q.w;
+  }
+  if constexpr (sizeof(T) == 8) {
+    typedef double d2a8 __attribute__((ext_vector_type(2), aligned(8)));
+    const d2a8 q0 = *reinterpret_cast<const d2a8 *>(vals + kb);
+    const d2a8 q1 = *reinterpret_cast<const d2a8 *>(vals + kb + 2);
+    B.v[0] = q0.x, B.v[1] = q0.y, B.v[2] = q1.x, B.v[3] = q1.y;
+  } else {
+    typedef float f4a4 __attribute__((ext_vector_type(4), aligned(4)));
+    const f4a4 q = *reinterpret_cast<const f4a4 *>(vals + kb);
+    B.v[0] = q.x, B.v[1] = q.y, B.v[2] = q.z, B.v[3] = q.w;
   }
 }
 
@@ -183,7 +214,7 @@ __device__ __forceinline__ T finish_blk(const StreamBlk<T> &B, T *prod, int *row
   if (B.ns > 0) {
 #pragma unroll
     for (int u = 0; u < StreamBlk<T>::PER; ++u) {
-      const int k = threadIdx.x + u * kThreads;
+      const int k = (int)threadIdx.x * StreamBlk<T>::PER + u;
       prod[k < B.ns ? k : kSpmvNnz] = B.v[u] * B.g[u];
     }
   }

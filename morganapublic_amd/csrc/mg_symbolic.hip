@@ -317,7 +317,7 @@ __global__ void k_inc_count(int64_t n, const uint64_t *__restrict__ keys, uint64
 // kColClusters sets *fail and the system keeps int32 columns.
 __global__ __launch_bounds__(256) void k_col_compress(BlkDesc *__restrict__ desc,
                                                       const int32_t *__restrict__ colidx,
-                                                      uint16_t *__restrict__ ofs,
+                                                      uint16_t *__restrict__ ofs, int ncols,
                                                       int *__restrict__ fail) {
   constexpr int PER = kSpmvNnz / 256;
   __shared__ int red[4];
@@ -341,6 +341,11 @@ __global__ __launch_bounds__(256) void k_col_compress(BlkDesc *__restrict__ desc
     __syncthreads();
     m = min(min(red[0], red[1]), min(red[2], red[3]));
     __syncthreads();
+    // bases stay <= max(ncols - kColSpan, 0), so any 16-bit entry of the
+    // array decodes inside [0, ncols) against any block's table (the SpMV
+    // streams read up to 3 entries past a block's end without a clamp); the
+    // window [base, base + kColSpan) still holds m
+    if (m != INT32_MAX) m = min(m, max(ncols - kColSpan, 0));
     if (threadIdx.x == 0) desc[b].base[cl] = m == INT32_MAX ? 0 : m;
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
@@ -486,7 +491,10 @@ void build_pattern(Ctx *c, int sys) {
   inclusive_sum(c, head, id, ncon);
   S.nnz = read_back(c, id + ncon - 1);
   S.slot = dalloc<int32_t>(c, ncon);
-  S.colidx = dalloc<int32_t>(c, S.nnz);
+  // 8 entries of padding after each SpMV stream array: the streams read a
+  // lane's 4 entries at once (zeroed; never used as a column)
+  S.colidx = dalloc<int32_t>(c, S.nnz + 8);
+  MG_HIP(hipMemsetAsync(S.colidx + S.nnz, 0, 8 * sizeof(int32_t), c->stream));
   int32_t *cnt = dalloc<int32_t>(c, nrows + 1);
   S.rowptr = dalloc<int32_t>(c, nrows + 1);
   MG_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (nrows + 1), c->stream));
@@ -594,10 +602,12 @@ void build_pattern(Ctx *c, int sys) {
   }
   // 16-bit columns for the SpMV streams (MGFE_SPMV_COL32=1 keeps int32)
   if (S.n_sblk > 0 && env_int("MGFE_SPMV_COL32", 0) == 0) {
-    S.cofs = dalloc<uint16_t>(c, S.nnz);
+    S.cofs = dalloc<uint16_t>(c, S.nnz + 8);
+    MG_HIP(hipMemsetAsync(S.cofs + S.nnz, 0, 8 * sizeof(uint16_t), c->stream));
     int *fail = dalloc<int>(c, 1);
     MG_HIP(hipMemsetAsync(fail, 0, sizeof(int), c->stream));
-    k_col_compress<<<S.n_sblk, 256, 0, c->stream>>>(S.sdesc, S.colidx, S.cofs, fail);
+    k_col_compress<<<S.n_sblk, 256, 0, c->stream>>>(S.sdesc, S.colidx, S.cofs, (int)S.nrows,
+                                                    fail);
     MG_HIP(hipGetLastError());
     S.c16 = read_back(c, fail) == 0;
     dfree(c, fail);
@@ -609,8 +619,10 @@ void build_pattern(Ctx *c, int sys) {
   for (const RowSegment &sg : S.segs)
     MG_CHECK((sg.lds_slots + 256) * 8 <= 64 * 1024, MG_ERR_SIZE,
              "pattern: a 64-row block owns more than 7936 CSR entries");
-  S.vals64 = dalloc<double>(c, S.nnz);
-  S.vals32 = dalloc<float>(c, S.nnz);
+  S.vals64 = dalloc<double>(c, S.nnz + 8);
+  MG_HIP(hipMemsetAsync(S.vals64 + S.nnz, 0, 8 * sizeof(double), c->stream));
+  S.vals32 = dalloc<float>(c, S.nnz + 8);
+  MG_HIP(hipMemsetAsync(S.vals32 + S.nnz, 0, 8 * sizeof(float), c->stream));
   S.dinv64 = dalloc<double>(c, nrows);
   S.dinv32 = dalloc<float>(c, nrows);
   S.built = true;
