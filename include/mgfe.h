@@ -169,7 +169,8 @@ int mg_slot_map_get(mg_ctx *ctx, int sys, int32_t *slot);
  * accumulation in FP32).                                                    */
 int mg_assemble(mg_ctx *ctx, int sys, const mg_operator *op, int prec);
 int mg_values_get(mg_ctx *ctx, int sys, int prec, void *vals);
-/* device pointer of the assembled values (double* or float*)               */
+/* device pointer of the nnz assembled values (double* or float*), in CSR
+ * order, owned by the context until the pattern is rebuilt or destroyed     */
 int mg_values_device(mg_ctx *ctx, int sys, int prec, void **vals_device);
 
 /* ---- load vector and Dirichlet lifting -------------------------------------
