@@ -172,6 +172,12 @@ int mg_values_get(mg_ctx *ctx, int sys, int prec, void *vals);
 /* device pointer of the nnz assembled values (double* or float*), in CSR
  * order, owned by the context until the pattern is rebuilt or destroyed     */
 int mg_values_device(mg_ctx *ctx, int sys, int prec, void **vals_device);
+/* assembly sequence number of the values a system currently holds in `prec`
+ * (increasing over the context's assemblies; 0: none).  No counterpart in the
+ * reference (an Epetra matrix is a new object per buildEpetraCrs call): lets
+ * a caller that keeps a matrix across calls check that it has not been
+ * re-assembled in between, as the adjoint solve of a gradient must.         */
+int mg_assembly_seq(mg_ctx *ctx, int sys, int prec, uint64_t *seq);
 
 /* ---- load vector and Dirichlet lifting -------------------------------------
  * replaces traitsVectorBuilder::buildEpetraVector (traitsVectorBuilder.hpp:217-274)

@@ -42,6 +42,7 @@ LOAD_CONST, LOAD_P0, LOAD_NODAL = 0, 1, 2
 EXPORTS = (
     "mg_ctx_create", "mg_ctx_destroy", "mg_last_error", "mg_ctx_stream", "mg_ctx_synchronize",
     "mg_copy_device",
+    "mg_assembly_seq",
     "mg_mesh_set", "mg_field_set", "mg_field_set_device", "mg_dofmap_build", "mg_elem_dofs_get",
     "mg_edges_get", "mg_dirichlet_set", "mg_dirichlet_mask_get", "mg_pattern_build",
     "mg_pattern_get", "mg_slot_map_get", "mg_assemble", "mg_values_get", "mg_values_device",
@@ -125,6 +126,7 @@ def _declare(L):
         "mg_assemble": [_P, _I, C.POINTER(Operator), _I],
         "mg_values_get": [_P, _I, _I, _P],
         "mg_values_device": [_P, _I, _I, C.POINTER(_P)],
+        "mg_assembly_seq": [_P, _I, _I, C.POINTER(C.c_uint64)],
         "mg_spmv": [_P, _I, _I, _P, _P],
         "mg_spmv_device": [_P, _I, _I, _P, _P],
         "mg_pcg_solve": [_P, C.POINTER(SolverParams), _P, _P, C.POINTER(SolveInfo)],
@@ -296,6 +298,12 @@ class Context:
         v = np.zeros(nz, dtype=np.float64 if prec == PREC_F64 else np.float32)
         self._chk(self._L.mg_values_get(self.h, sys, prec, _ptr(v)))
         return v
+
+    def assembly_seq(self, sys: int, prec=PREC_F64) -> int:
+        """Sequence number of the assembly whose values `sys` holds (0: none)."""
+        q = C.c_uint64()
+        self._chk(self._L.mg_assembly_seq(self.h, sys, prec, C.byref(q)))
+        return q.value
 
     def values_device_ptr(self, sys: int, prec=PREC_F64) -> int:
         p = _P()

@@ -851,16 +851,36 @@ int mg_assemble(mg_ctx *c, int sys, const mg_operator *op, int prec) {
     MG_CHECK(op->mass_coeff == MG_COEFF_CONST || c->field[MG_FIELD_MASS], MG_ERR_STATE,
              "assemble: mass P0 field not set");
   }
+  OpKey key;
+  key.terms = op->terms;
+  if (op->terms & MG_TERM_STIFF) {
+    key.stiff_coeff = op->stiff_coeff;
+    key.stiff_rule = op->stiff_rule;
+    if (op->stiff_coeff == MG_COEFF_P0)
+      key.stiff_field = c->field_ver[MG_FIELD_STIFF];
+    else
+      key.stiff_const = op->stiff_const;
+  }
+  if (op->terms & MG_TERM_MASS) {
+    key.mass_coeff = op->mass_coeff;
+    key.mass_rule = op->mass_rule;
+    if (op->mass_coeff == MG_COEFF_P0)
+      key.mass_field = c->field_ver[MG_FIELD_MASS];
+    else
+      key.mass_const = op->mass_const;
+  }
   if (prec == MG_PREC_F64) {
     assemble_t<double>(c, sys, op, prec);
     S.has64 = true;
     S.dinv64_ok = false;
     S.seq64 = ++c->asm_seq;
+    S.key64 = key;
   } else {
     assemble_t<float>(c, sys, op, prec);
     S.has32 = true;
     S.dinv32_ok = false;
     S.seq32 = ++c->asm_seq;
+    S.key32 = key;
   }
   API_END(c)
 }
@@ -884,15 +904,12 @@ int mg_adjoint_grad_k(mg_ctx *c, int sys, int rule, const double *u, const doubl
   MG_CHECK(c->dofs, MG_ERR_STATE, "adjoint: dof map not built");
   MG_CHECK(sys == MG_SYS_FULL || c->dirichlet_ok, MG_ERR_STATE, "adjoint: no interior numbering");
   const int64_t n = sys == MG_SYS_FULL ? c->ndofs : c->nfree;
-  double *du = dalloc<double>(c, n), *dl = dalloc<double>(c, n), *dg = dalloc<double>(c, c->nT);
+  DevTmp<double> du(c, n), dl(c, n), dg(c, c->nT);
   MG_HIP(hipMemcpyAsync(du, u, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   MG_HIP(hipMemcpyAsync(dl, lam, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   adjoint_launch(c, sys, rule, du, dl, dg);
   MG_HIP(hipMemcpyAsync(dJdk, dg, sizeof(double) * c->nT, hipMemcpyDeviceToHost, c->stream));
   MG_HIP(hipStreamSynchronize(c->stream));
-  dfree(c, du);
-  dfree(c, dl);
-  dfree(c, dg);
   API_END(c)
 }
 

@@ -393,6 +393,7 @@ int mg_field_set(mg_ctx *c, int which, const double *values) {
   if (!c->field[which]) c->field[which] = dalloc<double>(c, c->nT);
   MG_HIP(hipMemcpyAsync(c->field[which], values, sizeof(double) * c->nT, hipMemcpyHostToDevice,
                         c->stream));
+  ++c->field_ver[which];
   MG_HIP(hipStreamSynchronize(c->stream));
   API_END(c)
 }
@@ -405,6 +406,7 @@ int mg_field_set_device(mg_ctx *c, int which, const double *values) {
   if (!c->field[which]) c->field[which] = dalloc<double>(c, c->nT);
   MG_HIP(hipMemcpyAsync(c->field[which], values, sizeof(double) * c->nT,
                         hipMemcpyDeviceToDevice, c->stream));
+  ++c->field_ver[which];
   API_END(c)
 }
 
@@ -527,6 +529,16 @@ int mg_values_get(mg_ctx *c, int sys, int prec, void *vals) {
                           c->stream));
   }
   MG_HIP(hipStreamSynchronize(c->stream));
+  API_END(c)
+}
+
+int mg_assembly_seq(mg_ctx *c, int sys, int prec, uint64_t *seq) {
+  API_BEGIN(c)
+  MG_CHECK(sys == MG_SYS_FULL || sys == MG_SYS_INTERIOR, MG_ERR_ARG, "bad system");
+  MG_CHECK(prec == MG_PREC_F64 || prec == MG_PREC_F32, MG_ERR_ARG, "bad precision");
+  MG_CHECK(seq, MG_ERR_ARG, "null output");
+  const System &S = c->sys[sys];
+  *seq = prec == MG_PREC_F64 ? (S.has64 ? S.seq64 : 0) : (S.has32 ? S.seq32 : 0);
   API_END(c)
 }
 

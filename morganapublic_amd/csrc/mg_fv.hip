@@ -455,9 +455,22 @@ struct DDBuffers {
   uint8_t *fixed;
 };
 
-static DDBuffers dd_upload(Ctx *c, const double *eps, const double *rho, const double *mu,
-                           const double *phi, const double *dens, const uint8_t *fixed) {
-  DDBuffers B;
+// device copies of the host-side inputs (and outputs) of mg_dd_residual /
+// mg_dd_vjp, released with the scope
+struct DDHost : DDBuffers {
+  Ctx *c;
+  explicit DDHost(Ctx *c_) : DDBuffers{}, c(c_) {}
+  ~DDHost() {
+    for (void *p : {(void *)eps, (void *)rho, (void *)mu, (void *)ge, (void *)gm, (void *)phi,
+                    (void *)dens, (void *)rphi, (void *)rn, (void *)partial, (void *)fixed})
+      dfree(c, p);
+  }
+  DDHost(const DDHost &) = delete;
+  DDHost &operator=(const DDHost &) = delete;
+};
+
+static void dd_upload(Ctx *c, DDHost &B, const double *eps, const double *rho, const double *mu,
+                      const double *phi, const double *dens, const uint8_t *fixed) {
   const int64_t nT = c->nT, nV = c->nV;
   B.eps = dalloc<double>(c, nT);
   B.rho = dalloc<double>(c, nT);
@@ -476,14 +489,6 @@ static DDBuffers dd_upload(Ctx *c, const double *eps, const double *rho, const d
   MG_HIP(hipMemcpyAsync(B.phi, phi, sizeof(double) * nV, hipMemcpyHostToDevice, c->stream));
   MG_HIP(hipMemcpyAsync(B.dens, dens, sizeof(double) * nV, hipMemcpyHostToDevice, c->stream));
   MG_HIP(hipMemcpyAsync(B.fixed, fixed, nV, hipMemcpyHostToDevice, c->stream));
-  return B;
-}
-
-static void dd_free(Ctx *c, DDBuffers &B) {
-  for (void *p : {(void *)B.eps, (void *)B.rho, (void *)B.mu, (void *)B.ge, (void *)B.gm,
-                  (void *)B.phi, (void *)B.dens, (void *)B.rphi, (void *)B.rn, (void *)B.partial,
-                  (void *)B.fixed})
-    dfree(c, p);
 }
 
 // residual; with loss != nullptr also the loss sum(r_phi^2 + r_n^2), whose
@@ -554,7 +559,8 @@ int mg_dd_residual(mg_ctx *c, const double *eps, const double *rho, const double
   API_BEGIN(c)
   MG_CHECK(eps && rho && mu && phi && dens && fixed, MG_ERR_ARG, "dd: null input");
   MG_CHECK(c->dual_ok, MG_ERR_STATE, "dd: dual data not built");
-  DDBuffers B = dd_upload(c, eps, rho, mu, phi, dens, fixed);
+  DDHost B(c);
+  dd_upload(c, B, eps, rho, mu, phi, dens, fixed);
   double *dloss = B.partial + dd_gather_blocks(c);
   dd_residual_launch(c, B, loss ? dloss : nullptr);
   if (r_phi)
@@ -564,7 +570,6 @@ int mg_dd_residual(mg_ctx *c, const double *eps, const double *rho, const double
   if (loss)
     MG_HIP(hipMemcpyAsync(loss, dloss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   MG_HIP(hipStreamSynchronize(c->stream));
-  dd_free(c, B);
   API_END(c)
 }
 
@@ -575,7 +580,8 @@ int mg_dd_vjp(mg_ctx *c, const double *eps, const double *rho, const double *mu,
   MG_CHECK(eps && rho && mu && phi && dens && fixed && g_eps && g_mu, MG_ERR_ARG,
            "dd: null argument");
   MG_CHECK(c->dual_ok, MG_ERR_STATE, "dd: dual data not built");
-  DDBuffers B = dd_upload(c, eps, rho, mu, phi, dens, fixed);
+  DDHost B(c);
+  dd_upload(c, B, eps, rho, mu, phi, dens, fixed);
   dd_residual_launch(c, B);
   k_dd_vjp<<<blocks_for(c->nT, 128), 128, 0, c->stream>>>(c->nT, c->tets, c->xyz, c->sigma_el,
                                                           B.phi, B.dens, B.rphi, B.rn, B.ge, B.gm);
@@ -583,7 +589,6 @@ int mg_dd_vjp(mg_ctx *c, const double *eps, const double *rho, const double *mu,
   MG_HIP(hipMemcpyAsync(g_eps, B.ge, sizeof(double) * c->nT, hipMemcpyDeviceToHost, c->stream));
   MG_HIP(hipMemcpyAsync(g_mu, B.gm, sizeof(double) * c->nT, hipMemcpyDeviceToHost, c->stream));
   MG_HIP(hipStreamSynchronize(c->stream));
-  dd_free(c, B);
   API_END(c)
 }
 
@@ -623,14 +628,12 @@ int mg_bernoulli(mg_ctx *c, int64_t n, const double *x, double *y) {
   API_BEGIN(c)
   MG_CHECK(n >= 0 && (n == 0 || (x && y)), MG_ERR_ARG, "bernoulli: bad arguments");
   if (n == 0) return MG_OK;
-  double *dx = dalloc<double>(c, n), *dy = dalloc<double>(c, n);
+  DevTmp<double> dx(c, n), dy(c, n);
   MG_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   k_bernoulli<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, dx, dy);
   MG_HIP(hipGetLastError());
   MG_HIP(hipMemcpyAsync(y, dy, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   MG_HIP(hipStreamSynchronize(c->stream));
-  dfree(c, dx);
-  dfree(c, dy);
   API_END(c)
 }
 

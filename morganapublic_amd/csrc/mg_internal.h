@@ -63,6 +63,18 @@ struct alignas(64) BlkDesc {
   int32_t pad[4];
 };
 
+struct OpKey {
+  int terms = 0, stiff_coeff = 0, stiff_rule = 0, mass_coeff = 0, mass_rule = 0;
+  double stiff_const = 0.0, mass_const = 0.0;
+  uint64_t stiff_field = 0, mass_field = 0;  // field versions (0: constant coefficient)
+  bool operator==(const OpKey &o) const {
+    return terms == o.terms && stiff_coeff == o.stiff_coeff && stiff_rule == o.stiff_rule &&
+           mass_coeff == o.mass_coeff && mass_rule == o.mass_rule &&
+           stiff_const == o.stiff_const && mass_const == o.mass_const &&
+           stiff_field == o.stiff_field && mass_field == o.mass_field;
+  }
+};
+
 struct System {
   bool built = false;
   int64_t nrows = 0, nnz = 0, n_inc = 0;
@@ -89,6 +101,10 @@ struct System {
   // assembly sequence numbers of the two value arrays: the mixed solvers
   // refresh the FP32 copy from FP64 values assembled after it
   uint64_t seq64 = 0, seq32 = 0;
+  // what each value array holds (operator, rules, constants and the versions
+  // of the P0 fields it read): the FP32 array stands in for the FP64 one in
+  // the mixed solvers only when both hold the same operator
+  OpKey key64, key32;
   double *dinv64 = nullptr;
   float *dinv32 = nullptr;
   bool dinv64_ok = false, dinv32_ok = false;
@@ -107,6 +123,7 @@ struct Ctx {
   int32_t *btri = nullptr; // 3 nB, 0-based
   int32_t *bgeo = nullptr; // nB
   double *field[2] = {nullptr, nullptr};
+  uint64_t field_ver[2] = {0, 0};  // bumped by every mg_field_set / mg_field_set_device
 
   // dof map
   int order = 0, nloc = 0;
@@ -174,6 +191,34 @@ T *dalloc(Ctx *c, size_t n) {
   return static_cast<T *>(p);
 }
 void dfree(Ctx *c, void *p);
+
+// A device temporary of a host-copy entry point: released when it goes out of
+// scope, also when the call ends early through MG_CHECK / MG_HIP.
+template <typename T>
+struct DevTmp {
+  Ctx *c;
+  T *p;
+  DevTmp(Ctx *c_, size_t n) : c(c_), p(dalloc<T>(c_, n)) {}
+  ~DevTmp() { dfree(c, p); }
+  DevTmp(const DevTmp &) = delete;
+  DevTmp &operator=(const DevTmp &) = delete;
+  operator T *() const { return p; }
+};
+
+// A pair of HIP events destroyed with the scope.
+struct EventPair {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  EventPair() {
+    if (hipEventCreate(&e0) != hipSuccess) e0 = nullptr;
+    if (hipEventCreate(&e1) != hipSuccess) e1 = nullptr;
+  }
+  ~EventPair() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+  EventPair(const EventPair &) = delete;
+  EventPair &operator=(const EventPair &) = delete;
+};
 void free_system(Ctx *c, System &S);  // releases a system's buffers and resets it
 void *scratch(Ctx *c, size_t bytes);  // context-owned reusable temporary
 

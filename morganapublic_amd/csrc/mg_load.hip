@@ -164,13 +164,11 @@ int mg_assemble_load(mg_ctx *c, int sys, int kind, const double *f, int64_t nf, 
   MG_CHECK(f && F, MG_ERR_ARG, "load: null pointer");
   MG_CHECK(nf == load_len(c, kind), MG_ERR_ARG, "load: f has the wrong length");
   const int64_t n = c->sys[sys].nrows;
-  double *df = dalloc<double>(c, nf), *dF = dalloc<double>(c, n);
+  DevTmp<double> df(c, nf), dF(c, n);
   MG_HIP(hipMemcpyAsync(df, f, sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
   load_launch(c, sys, kind, df, rule, dF);
   MG_HIP(hipMemcpyAsync(F, dF, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   MG_HIP(hipStreamSynchronize(c->stream));
-  dfree(c, df);
-  dfree(c, dF);
   API_END(c)
 }
 
@@ -189,14 +187,12 @@ int mg_dirichlet_lift(mg_ctx *c, const double *g, double *b) {
   MG_CHECK(c->dirichlet_ok, MG_ERR_STATE, "lift: Dirichlet dofs not set");
   MG_CHECK(c->sys[MG_SYS_FULL].built && c->sys[MG_SYS_FULL].has64, MG_ERR_STATE,
            "lift: full FP64 matrix not assembled");
-  double *dg = dalloc<double>(c, c->ndofs), *db = dalloc<double>(c, c->nfree);
+  DevTmp<double> dg(c, c->ndofs), db(c, c->nfree);
   MG_HIP(hipMemcpyAsync(dg, g, sizeof(double) * c->ndofs, hipMemcpyHostToDevice, c->stream));
   MG_HIP(hipMemcpyAsync(db, b, sizeof(double) * c->nfree, hipMemcpyHostToDevice, c->stream));
   lift_launch(c, dg, db);
   MG_HIP(hipMemcpyAsync(b, db, sizeof(double) * c->nfree, hipMemcpyDeviceToHost, c->stream));
   MG_HIP(hipStreamSynchronize(c->stream));
-  dfree(c, dg);
-  dfree(c, db);
   API_END(c)
 }
 

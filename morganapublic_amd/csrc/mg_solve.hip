@@ -882,11 +882,12 @@ static double true_relres(Ctx *c, System &S, const double *b, const double *x, W
 // recent as the FP64 one (the two assemblies of one operator), otherwise the
 // FP64 values rounded, so an FP64 re-assembly never leaves a stale FP32 copy.
 static void refresh_f32(Ctx *c, System &S) {
-  if (S.has32 && S.seq32 >= S.seq64) return;
+  if (S.has32 && S.seq32 >= S.seq64 && S.key32 == S.key64) return;
   k_vals_to_f32<<<blocks_for(S.nnz, 256), 256, 0, c->stream>>>(S.nnz, S.vals64, S.vals32);
   MG_HIP(hipGetLastError());
   S.has32 = true;
   S.seq32 = S.seq64;
+  S.key32 = S.key64;
   S.dinv32_ok = false;
 }
 
@@ -1005,10 +1006,9 @@ static void pcg_solve(Ctx *c, const mg_solver_params *prm, const double *b, doub
   MG_CHECK(S.built && S.has64, MG_ERR_STATE, "solve: interior FP64 matrix not assembled");
   const int64_t n = S.nrows;
   Workspace w = workspace(c, n);
-  hipEvent_t e0, e1;
-  MG_HIP(hipEventCreate(&e0));
-  MG_HIP(hipEventCreate(&e1));
-  MG_HIP(hipEventRecord(e0, c->stream));
+  EventPair ev;
+  MG_CHECK(ev.e0 && ev.e1, MG_ERR_HIP, "solve: hipEventCreate failed");
+  MG_HIP(hipEventRecord(ev.e0, c->stream));
   ensure_dinv(c, MG_SYS_INTERIOR, MG_PREC_F64);
   // ||b||
   {
@@ -1086,12 +1086,10 @@ static void pcg_solve(Ctx *c, const mg_solver_params *prm, const double *b, doub
       MG_HIP(hipGetLastError());
     }
   }
-  MG_HIP(hipEventRecord(e1, c->stream));
-  MG_HIP(hipEventSynchronize(e1));
+  MG_HIP(hipEventRecord(ev.e1, c->stream));
+  MG_HIP(hipEventSynchronize(ev.e1));
   float ms = 0.f;
-  MG_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  MG_HIP(hipEventElapsedTime(&ms, ev.e0, ev.e1));
   if (info) {
     info->status = status;
     info->iters = iters;
@@ -1143,24 +1141,20 @@ int mg_spmv(mg_ctx *c, int sys, int prec, const double *x, double *y) {
            "spmv: matrix not assembled");
   const int64_t n = S.nrows;
   if (prec == MG_PREC_F64) {
-    double *dx = dalloc<double>(c, n), *dy = dalloc<double>(c, n);
+    DevTmp<double> dx(c, n), dy(c, n);
     MG_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     spmv_launch(c, sys, prec, dx, dy);
     MG_HIP(hipMemcpyAsync(y, dy, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     MG_HIP(hipStreamSynchronize(c->stream));
-    dfree(c, dx);
-    dfree(c, dy);
   } else {
     std::vector<float> hx(n), hy(n);
     for (int64_t i = 0; i < n; ++i) hx[i] = (float)x[i];
-    float *dx = dalloc<float>(c, n), *dy = dalloc<float>(c, n);
+    DevTmp<float> dx(c, n), dy(c, n);
     MG_HIP(hipMemcpyAsync(dx, hx.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
     spmv_launch(c, sys, prec, dx, dy);
     MG_HIP(hipMemcpyAsync(hy.data(), dy, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
     MG_HIP(hipStreamSynchronize(c->stream));
     for (int64_t i = 0; i < n; ++i) y[i] = hy[i];
-    dfree(c, dx);
-    dfree(c, dy);
   }
   API_END(c)
 }
@@ -1172,11 +1166,14 @@ int mg_pcg_solve_device(mg_ctx *c, const mg_solver_params *p, const double *b, d
   MG_CHECK(p->mode == MG_SOLVE_F64 || p->mode == MG_SOLVE_F32_REFINE ||
                p->mode == MG_SOLVE_F32_RELIABLE,
            MG_ERR_ARG, "solve: bad mode");
-  pcg_solve(c, p, b, x, info);
-  if (info && info->status != MG_OK) {
-    c->err = info->status == MG_ERR_NOCONV ? "solve: iteration limit reached"
-                                           : "solve: CG breakdown (p.Ap <= 0 or NaN)";
-    return info->status;
+  // the status is returned whether or not the caller asked for the info
+  mg_solve_info loc{};
+  pcg_solve(c, p, b, x, &loc);
+  if (info) *info = loc;
+  if (loc.status != MG_OK) {
+    c->err = loc.status == MG_ERR_NOCONV ? "solve: iteration limit reached"
+                                         : "solve: CG breakdown (p.Ap <= 0 or NaN)";
+    return loc.status;
   }
   API_END(c)
 }
@@ -1189,9 +1186,9 @@ int mg_pcg_solve(mg_ctx *c, const mg_solver_params *p, const double *b, double *
                p->mode == MG_SOLVE_F32_RELIABLE,
            MG_ERR_ARG, "solve: bad mode");
   System &S = c->sys[MG_SYS_INTERIOR];
-  MG_CHECK(S.built, MG_ERR_STATE, "solve: interior pattern not built");
+  MG_CHECK(S.built && S.has64, MG_ERR_STATE, "solve: interior FP64 matrix not assembled");
   const int64_t n = S.nrows;
-  double *db = dalloc<double>(c, n), *dx = dalloc<double>(c, n);
+  DevTmp<double> db(c, n), dx(c, n);
   MG_HIP(hipMemcpyAsync(db, b, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   if (p->use_x0)
     MG_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
@@ -1199,8 +1196,6 @@ int mg_pcg_solve(mg_ctx *c, const mg_solver_params *p, const double *b, double *
   pcg_solve(c, p, db, dx, &loc);
   MG_HIP(hipMemcpyAsync(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   MG_HIP(hipStreamSynchronize(c->stream));
-  dfree(c, db);
-  dfree(c, dx);
   if (info) *info = loc;
   if (loc.status != MG_OK) {
     c->err = loc.status == MG_ERR_NOCONV ? "solve: iteration limit reached"

@@ -79,3 +79,28 @@ def test_grad_wrt_rhs_is_adjoint(ctx):
     A = sp.csr_matrix((va, ci, rp), shape=(rp.size - 1, rp.size - 1))
     lam = spla.spsolve(A.T.tocsc(), uobs.cpu().numpy())
     np.testing.assert_allclose(ff.grad.cpu().numpy(), lam, rtol=1e-8, atol=1e-12 * np.abs(lam).max())
+
+
+@pytest.mark.parametrize("mode", [L.SOLVE_F64, L.SOLVE_F32_RELIABLE])
+def test_backward_after_another_forward_uses_its_own_operator(ctx, mode):
+    # forward(k1), forward(k2), then backward of the first: the context holds
+    # A(k2) by then; the gradient must still be the one at k1
+    from morganapublic_amd.autograd import diffusion_solve
+    p, f, uobs, k1 = _problem(ctx, 1, 6)
+    k2 = 2.5 * k1 + 0.5
+
+    def grad_alone(k):
+        kk = k.clone().requires_grad_(True)
+        u = diffusion_solve(kk, f, ctx, mode=mode, rule=1, tol=1e-12)
+        ((u - uobs) ** 2).sum().backward()
+        return kk.grad.cpu().numpy()
+
+    g1, g2 = grad_alone(k1), grad_alone(k2)
+    assert np.linalg.norm(g1 - g2) > 1e-3 * np.linalg.norm(g1)
+    ka, kb = k1.clone().requires_grad_(True), k2.clone().requires_grad_(True)
+    ua = diffusion_solve(ka, f, ctx, mode=mode, rule=1, tol=1e-12)
+    ub = diffusion_solve(kb, f, ctx, mode=mode, rule=1, tol=1e-12)
+    ((ua - uobs) ** 2).sum().backward()
+    ((ub - uobs) ** 2).sum().backward()
+    for got, want in ((ka.grad.cpu().numpy(), g1), (kb.grad.cpu().numpy(), g2)):
+        assert np.linalg.norm(got - want) <= 1e-8 * np.linalg.norm(want)
