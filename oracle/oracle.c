@@ -15,8 +15,13 @@
  *   PCG           scipy.sparse.linalg.spsolve on the same system
  *   dual FV data  sum(cellVolume) = |cube|, cellVolume(v) = sum vol(T)/4,
  *                 per-element split sums to edgeSurf
- *   SG residual   Bernoulli identities, linear-profile closed form
- *   gradients     central finite differences of the oracle's own loss
+ *   SG residual   Bernoulli identities, n = e^phi equilibrium, and the flux
+ *                 of each edge against the constant current of the 1-D
+ *                 drift-diffusion problem along it, integrated by RK4
+ *                 without B, through the brute-forced dual facet area
+ *   gradients     central finite differences of the oracle's own loss;
+ *                 or_adjoint_grad_k by sum_e k_e g_e = -lam^T A(k) u and
+ *                 g_i = -lam^T A(1_i) u with or_assemble's matrices
  *   load vector   sum F = |cube|; P1 vol/4, P2 -vol/20 (vertex) and vol/5
  *                 (edge) closed forms; F = M f with the independently
  *                 assembled mass matrix; linearity in f

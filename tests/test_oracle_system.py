@@ -255,3 +255,83 @@ def test_reference_mesh_fixtures_load_and_assemble():
         N = rp.size - 1
         A = sp.csr_matrix((va, ci, rp), shape=(N, N))
         assert np.abs(A @ np.ones(N)).max() < 1e-13
+
+
+def _rk4_linear(a, L, y0, rhs, steps=4000):
+    """y(L) for y' = a y + rhs, y(0) = y0, by classical RK4 (no exponential)."""
+    h = L / steps
+    y = y0
+    for _ in range(steps):
+        k1 = a * y + rhs
+        k2 = a * (y + 0.5 * h * k1) + rhs
+        k3 = a * (y + 0.5 * h * k2) + rhs
+        k4 = a * (y + h * k3) + rhs
+        y += h / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+    return y
+
+
+def test_sg_flux_is_the_constant_current_of_the_1d_problem():
+    # Along an edge a -> b of length L with phi linear from phi_a to phi_b, the
+    # electron density with constant current j solves n' - phi' n = j, n(0) =
+    # n_a, n(L) = n_b; the flux through the edge's dual facet is mu sigma j.
+    # j is found here by integrating that ODE (RK4, no Bernoulli function), and
+    # sigma = |N| / 2 from the facet triangles of the tet (a one-element mesh:
+    # the aggregated normal is the element's own).  The oracle's residual must
+    # be the sum of these fluxes: a slip of 1/L^2 for 1/L, 2 sigma for sigma,
+    # or a swapped B(dphi) / B(-dphi) fails.
+    P = np.array([[0.05, -0.02, 0.01], [1.1, 0.1, -0.05], [0.2, 0.9, 0.1], [-0.1, 0.15, 1.2]])
+    m = W.TetMesh(n=1, xyz=P, tets=np.array([[1, 2, 3, 4]], np.int32),
+                  btri=np.zeros((0, 3), np.int32), bgeo=np.zeros(0, np.int32))
+    ee, en = O.edges(m)
+    d = O.dual_fv(m, ee, en)
+    phi = np.array([0.3, -1.7, 2.2, 0.9])
+    dens = np.array([1.5, 0.4, 2.5, 0.8])
+    mu = np.array([1.3])
+    eps, rho = np.ones(1), np.zeros(1)
+    fixed = np.zeros(4, np.uint8)
+    _, _, rn = O.dd_loss(m, d, eps, rho, mu, phi, dens, fixed, ee, en)
+    BA = P.mean(axis=0)
+    want = np.zeros(4)
+    for a, b in [(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)]:
+        t = P[b] - P[a]
+        L = np.linalg.norm(t)
+        E = 0.5 * (P[a] + P[b])
+        N = np.zeros(3)
+        for c in range(4):
+            if c in (a, b):
+                continue
+            F = (P[a] + P[b] + P[c]) / 3.0
+            n = np.cross(F - E, BA - E)
+            N += n if n @ t >= 0 else -n
+        sigma = 0.5 * np.linalg.norm(N)
+        g = (phi[b] - phi[a]) / L
+        E1 = _rk4_linear(g, L, 1.0, 0.0)   # homogeneous part, n_a = 1
+        I1 = _rk4_linear(g, L, 0.0, 1.0)   # response to a unit current
+        j = (dens[b] - dens[a] * E1) / I1
+        want[a] += mu[0] * sigma * j
+        want[b] -= mu[0] * sigma * j
+    np.testing.assert_allclose(rn, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("order,rule", [(1, 1), (2, 4), (2, 11)])
+def test_adjoint_element_gradient_against_assembled_operator(order, rule):
+    # A(k) = sum_e k_e Khat_e, so dJ/dk_e = -lam^T Khat_e u is fixed by the
+    # assembly: summed against k it is -lam^T A(k) u, and with k the
+    # indicator of element i it is -lam^T A(1_i) u (both sides through
+    # or_assemble, an independent code path)
+    m = W.kuhn_cube(2, jitter=0.2, seed=3)
+    rng = np.random.default_rng(1)
+    k = np.exp(rng.normal(size=m.nT))
+    r, c, v = O.assemble(order, m, O.OP_STIFF, c_stiff=k, rule_stiff=rule)
+    N = r.size - 1
+    A = sp.csr_matrix((v, c, r), shape=(N, N))
+    u, lam = rng.normal(size=N), rng.normal(size=N)
+    dofs = O.elem_dofs(order, m)
+    g = O.adjoint_grad_k(order, m, dofs, u, lam, rule=rule)
+    assert abs(k @ g + lam @ (A @ u)) <= 1e-12 * np.abs(lam) @ (abs(A) @ np.abs(u))
+    for i in rng.choice(m.nT, 4, replace=False):
+        ki = np.zeros(m.nT)
+        ki[i] = 1.0
+        r, c, v = O.assemble(order, m, O.OP_STIFF, c_stiff=ki, rule_stiff=rule)
+        Ai = sp.csr_matrix((v, c, r), shape=(N, N))
+        assert abs(g[i] + lam @ (Ai @ u)) <= 1e-12 * max(1.0, abs(g[i]))
