@@ -103,25 +103,42 @@ def bytes_pcg_iter(n, nnz, vb):
     return nnz * (vb + 4) + 4 * (n + 1) + 13 * vb * n
 
 
+def bytes_adjoint(T, V, nloc, n_sys, n_full):
+    """Algorithmic bytes of the adjoint element kernel: per element its 4
+    vertex ids and nloc dof ids, one gradient written; coordinates once; u and
+    lambda once each (system numbering) and the dof -> system map once."""
+    return T * (16 + 4 * nloc + 8) + 24 * V + 16 * n_sys + 4 * n_full
+
+
 def bytes_spmv(n, nnz, vb):
     """Algorithmic bytes of y = A x: values and 4-byte columns once, row
     pointers, x and y once."""
     return nnz * (vb + 4) + 4 * (n + 1) + 2 * vb * n
 
 
+def kernel_source_sha(name="mg_assemble.hip"):
+    import hashlib
+    with open(os.path.join(ROOT, "morganapublic_amd", "csrc", name), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
+
+
 def measured_traffic(n):
-    """HBM bytes of one FP64 assembly at this size from the committed counter
-    summary (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the copy kernel's
-    reading); None when no summary for this size exists."""
+    """L2 <-> memory bytes of one FP64 assembly at this size from the committed
+    counter summary (rocprofv3 TCC_EA0_RDREQ_{32B,64B,128B} and
+    TCC_EA0_WRREQ(_64B), summed over the assembly's kernels), or None when
+    no summary exists or it was taken from another version of the assembly
+    kernels (its source hash differs from the file built here)."""
     path = os.path.join(ROOT, "profiles", "assembly_p2_%d_traffic.json" % n)
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
+    if d.get("source_sha") != kernel_source_sha():
+        return None
     return d.get("bytes_per_assembly")
 
 
-def bench_config5(ctx, timer, n=128):
+def bench_config5(ctx, timer, keep, n=128):
     """128^3 P1, lognormal k: forward + adjoint solve with both FP32 + FP64
     refinement methods to 1e-10, vs the FP64 plain method; dJ/dk for
     J = ||u - u_obs||^2."""
@@ -150,6 +167,8 @@ def bench_config5(ctx, timer, n=128):
         out["assembly_%s_us" % name] = round(1e3 * ms, 1)
         if prec == L.PREC_F64:
             out["assembly_f64_roofline_frac"] = round(b_asm / (ms * 1e-3) / 8e12, 4)
+            rp5, ci5 = ctx.pattern(L.SYS_INTERIOR)
+            keep["p1_128"] = (p, rp5, ci5, ctx.values(L.SYS_INTERIOR), ctx.dirichlet_mask())
         log("cfg5 P1 assembly %s: %.1f us (%.0f M elem/s)" % (name, 1e3 * ms, m.nT / ms / 1e3))
     grads = {}
     for name, mode in (("f64", L.SOLVE_F64), ("f32_refine", L.SOLVE_F32_REFINE),
@@ -182,6 +201,16 @@ def bench_config5(ctx, timer, n=128):
     for ptr in (db, du, dl, dg):
         timer.free(ptr)
     return out
+
+
+def bytes_dd(nT, nV, vjp):
+    """Algorithmic bytes of the drift-diffusion residual (+ VJP): per element
+    the 4 vertex ids, eps, rho, mu and the 6 per-element dual areas once; per
+    vertex its coordinates, phi, n and the fixed flag once, and the two
+    residuals written once; the VJP also writes d/d(eps, mu) per element (its
+    residuals are recomputed, not re-read)."""
+    b = nT * (16 + 24 + 48) + nV * (24 + 16 + 1 + 16)
+    return b + (nT * 16 if vjp else 0)
 
 
 def bench_config4(ctx, timer, n=96, reps=5):
@@ -218,8 +247,12 @@ def bench_config4(ctx, timer, n=96, reps=5):
         % (t_dual * 1e3, ms_res, ms_vjp))
     for ptr in args + [rp, rn, loss, ge, gm]:
         timer.free(ptr)
+    b_res, b_vjp = bytes_dd(m.nT, m.nV, False), bytes_dd(m.nT, m.nV, True)
     return {"tets": m.nT, "vertices": m.nV, "dual_build_ms": round(t_dual * 1e3, 2),
-            "residual_loss_ms": round(ms_res, 3), "residual_vjp_ms": round(ms_vjp, 3)}
+            "residual_loss_ms": round(ms_res, 3), "residual_vjp_ms": round(ms_vjp, 3),
+            "residual_loss_bytes": b_res, "residual_vjp_bytes": b_vjp,
+            "residual_loss_roofline_frac": round(b_res / (ms_res * 1e-3) / 8e12, 4),
+            "residual_vjp_roofline_frac": round(b_vjp / (ms_vjp * 1e-3) / 8e12, 4)}
 
 
 def _cpu_model():
@@ -230,6 +263,66 @@ def _cpu_model():
     except OSError:
         pass
     return "unknown CPU"
+
+
+def oracle_parity(keep, threads):
+    """Max row-scaled difference between the GPU's FP64 values and the
+    oracle's element matrices added into the same pattern on host threads
+    (O.accumulate), at the full 64^3 P2 and 128^3 P1 sizes; the oracle is
+    the checker here, the values checked are the ones the timed steps made."""
+    import oracle as O
+    out = {}
+
+    def rowscaled(rp, got, want):
+        rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+        sc = np.zeros(rp.size - 1)
+        np.maximum.at(sc, rows, np.abs(want))
+        sc[sc == 0] = np.abs(want).max()
+        return float((np.abs(got - want) / sc[rows]).max())
+
+    if "p2_64" in keep:
+        p, rp, ci, got = keep["p2_64"]
+        d = O.elem_dofs(2, p.mesh)
+        want, miss = O.accumulate(2, p.mesh, O.OP_STIFF, d, rp, ci, c_stiff=p.k, rule_stiff=4,
+                                  threads=threads)
+        out["p2_64_full_rowscaled_max"] = rowscaled(rp, got, want) if miss == 0 else None
+        out["p2_64_full_missing_pairs"] = int(miss)
+    if "p1_128" in keep:
+        p, rp, ci, got, fixed = keep["p1_128"]
+        sysmap = np.where(fixed == 0, np.cumsum(fixed == 0) - 1, -1).astype(np.int32)
+        d = sysmap[O.elem_dofs(1, p.mesh)]
+        want, miss = O.accumulate(1, p.mesh, O.OP_STIFF, d, rp, ci, c_stiff=p.k, rule_stiff=1,
+                                  threads=threads)
+        out["p1_128_interior_rowscaled_max"] = rowscaled(rp, got, want) if miss == 0 else None
+        out["p1_128_interior_missing_pairs"] = int(miss)
+    out["threads"] = threads
+    log("oracle parity at full size: %s" % out)
+    return out
+
+
+def cpu_pcg_baseline(keep, threads, gpu_iters):
+    """The oracle's Jacobi-PCG iteration (or_pcg_fixed) on the 64^3 P2
+    interior system, SpMV and vector updates split over host threads; timed
+    per iteration (a fixed count from x = 0), on all threads and on one, and
+    scaled to the FP64 GPU solve's iteration count to 1e-10."""
+    import oracle as O
+    if "p2_64_int" not in keep:
+        return None
+    rp, ci, va, b = keep["p2_64_int"]
+    _, t_n, _ = O.pcg_fixed(rp, ci, va, b, 20, threads=threads)
+    _, t_1, _ = O.pcg_fixed(rp, ci, va, b, 3, threads=1)
+    out = {"ms_per_iter": round(t_n * 1e3, 3), "threads": threads,
+           "ms_per_iter_1core": round(t_1 * 1e3, 3),
+           "iters_timed": 20, "gpu_f64_iters": gpu_iters,
+           "est_ms_to_1e-10": round(t_n * 1e3 * gpu_iters, 1) if gpu_iters else None,
+           "kind": "port",
+           "sample": "oracle Jacobi-PCG iteration (or_pcg_fixed) on the 64^3 P2 interior system "
+                     "(N = %d, nnz = %d): 20 iterations on %d threads, 3 on one core; the "
+                     "time to 1e-10 is the per-iteration time x the GPU FP64 solve's iteration "
+                     "count" % (b.size, va.size, threads)}
+    log("CPU PCG baseline: %.2f ms/iteration on %d threads (%.2f on one core)"
+        % (t_n * 1e3, threads, t_1 * 1e3))
+    return out
 
 
 def cpu_baseline(problem, budget_s=20.0, max_threads=16):
@@ -306,10 +399,21 @@ def main():
     log("symbolic (edges, dofs, 2 patterns, slot maps, incidence) %.3fs: N=%d nnz=%d N_int=%d "
         "nnz_int=%d" % (t_sym, ndofs, nnz, nint, nnz_int))
     ctx.field_set(L.FIELD_STIFF, p.k)
+    keep = {}
 
     def assemble(sysid=L.SYS_FULL, prec=L.PREC_F64):
         ctx.assemble(sysid, L.TERM_STIFF, stiff_coeff=L.COEFF_P0, stiff_rule=11, prec=prec)
 
+    # device settle before the warm-up steps: the same step, untimed, for
+    # about 0.3 s (the first ~60 steps after the symbolic phase run up to 10%
+    # slower while the device clocks come up; a few warm-up steps do not
+    # cover that)
+    settle, t_settle = 0, time.perf_counter()
+    while time.perf_counter() - t_settle < 0.3 or settle < 50:
+        for _ in range(10):
+            assemble()
+        settle += 10
+        timer.sync()
     for _ in range(args.warmup):
         assemble()
     timer.sync()
@@ -326,6 +430,9 @@ def main():
         % (ms_step, wall * 1e3 / args.steps, value))
     b_asm = bytes_assembly(m.nT, m.nV, 10, nnz, 8)
     achieved = b_asm / (ms_step * 1e-3) / 1e9
+    if not args.no_cpu:
+        rp_f, ci_f = ctx.pattern(L.SYS_FULL)
+        keep["p2_64"] = (p, rp_f, ci_f, ctx.values(L.SYS_FULL))
 
     # attainable HBM bandwidth: a plain 1 GiB device copy on the same stream
     cbytes = 1 << 30
@@ -404,7 +511,9 @@ def main():
         rmax = np.zeros(nint)
         np.maximum.at(rmax, rows, np.abs(v64))
         mat_err = float((np.abs(v32.astype(np.float64) - v64) / rmax[rows]).max())
-        del v64, v32, rows
+        if not args.no_cpu:
+            keep["p2_64_int"] = (rp_i, ctx.pattern(L.SYS_INTERIOR)[1], v64, b)
+        del v32, rows
         ctx.pcg_device(db, dx, mode=L.SOLVE_F64, tol=1e-10)
         x64 = timer.to_host(dx, nint)
         ctx.pcg_device(db, dx, mode=mmode, tol=1e-10)
@@ -431,27 +540,48 @@ def main():
         ctx.adjoint_grad_k_device(L.SYS_INTERIOR, du, dlam, dg, rule=11)
         ms_grad = timer.stop_ms()
         log("forward %.2f ms, gradient %.2f ms, ratio %.3f" % (ms_fwd, ms_grad, ms_grad / ms_fwd))
+        # the adjoint element kernel alone (dJ/dk_e = -lam_e^T Khat_e u_e)
+        timer.sync()
+        timer.start()
+        for _ in range(10):
+            ctx.adjoint_grad_k_device(L.SYS_INTERIOR, du, dlam, dg, rule=11)
+        us_adj = timer.stop_ms() / 10 * 1e3
+        b_adj = bytes_adjoint(m.nT, m.nV, 10, nint, ndofs)
+        log("adjoint element kernel: %.1f us, %.0f GB/s on %d algorithmic bytes"
+            % (us_adj, b_adj / (us_adj * 1e-6) / 1e9, b_adj))
         it_bytes64 = bytes_pcg_iter(nint, nnz_int, 8)
         it_bytes32 = bytes_pcg_iter(nint, nnz_int, 4)
         out_extra = {
             "spmv_interior": spmv,
             "config3_errors": {"fp32_matrix_rowscaled_max": mat_err,
-                               "mixed_solution_rel_vs_fp64": sol_err},
+                               "mixed_solution_rel_vs_fp64": sol_err,
+                               "fp32_element_matrices": "closed form in FP32 VALU (no MFMA: "
+                                                        "no dense product is left once the "
+                                                        "quadrature is contracted)"},
             "pcg_to_1e-10": res,
             "pcg_mixed_method": mixed,
             "pcg_speedup_mixed_vs_f64": round(res["f64"]["ms"] / res[mixed]["ms"], 3),
             "pcg_speedup_f32_refine_vs_f64": round(res["f64"]["ms"] / res["f32_refine"]["ms"], 3),
             "pcg_speedup_f32_reliable_vs_f64": round(res["f64"]["ms"] / res["f32_reliable"]["ms"], 3),
             "grad_fwd_ratio": round(ms_grad / ms_fwd, 3),
+            "adjoint_kernel": {"us": round(us_adj, 1), "bytes": b_adj,
+                               "GBps": round(b_adj / (us_adj * 1e-6) / 1e9, 1),
+                               "roofline_frac": round(b_adj / (us_adj * 1e-6) / 8e12, 4)},
             "forward_ms": round(ms_fwd, 3), "gradient_ms": round(ms_grad, 3),
             "pcg_f64_GBps_per_iter": round(it_bytes64 * res["f64"]["iters"] / (res["f64"]["ms"] * 1e-3) / 1e9, 1),
             "pcg_f64_roofline_frac": round(it_bytes64 * res["f64"]["iters"] / (res["f64"]["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         }
 
     if not args.no_solve and not args.no_configs:
-        out_extra["config5_gradient_128"] = bench_config5(ctx, timer)
+        out_extra["config5_gradient_128"] = bench_config5(ctx, timer, keep if not args.no_cpu else {})
         out_extra["config4_drift_diffusion_96"] = bench_config4(ctx, timer)
-    cpu = None if args.no_cpu else cpu_baseline(p)
+    cpu = None
+    if not args.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline(p)
+        out_extra["oracle_parity"] = oracle_parity(keep, threads)
+        gpu_iters = out_extra.get("pcg_to_1e-10", {}).get("f64", {}).get("iters")
+        out_extra["cpu_pcg_baseline"] = cpu_pcg_baseline(keep, threads, gpu_iters)
     line = {
         "metric": "M elem/s assembly; PCG time to 1e-10 (FP32+refine vs FP64); grad/fwd ratio",
         "value": round(value, 2),
@@ -476,6 +606,7 @@ def main():
                      "kernel": "one FP64 assembly = k_elem_factors + k_asm_rows (vertex rows) + "
                                "k_asm_rows (edge rows); algorithmic bytes %d" % b_asm,
                      "copy_GBps": round(copy_gbs, 1)},
+        "settle_steps": settle,
         "assembly_f32_ms": round(ms_asm32, 4),
         "symbolic_s": round(t_sym, 3),
         "cpu_baseline": cpu,

@@ -36,6 +36,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -363,6 +364,52 @@ static int cmp_kv(const void *a, const void *b) {
   return x->idx < y->idx ? -1 : (x->idx > y->idx ? 1 : 0);
 }
 
+/* Adds the element matrices of elements [e0, e1) into vals on a given CSR     */
+/* pattern: the element loop of traitsMatrixBuilder::buildEpetraCrs          */
+/* (traitsMatrixBuilder.hpp:404-437) with InsertGlobalValues replaced by a    */
+/* binary search of each column in its row.  sdofs[n e + i] is the system     */
+/* index of element basis i (-1: not in the system, e.g. an eliminated dof).  */
+/* The checker of full-size assemblies: host threads each take a block of     */
+/* elements into their own vals.  Returns the number of (row, column) pairs   */
+/* the pattern lacks (0 when the pattern holds every element pair), or -1 for */
+/* a bad rule.                                                                */
+int64_t or_accumulate(int ops, int order, const double *xyz, const int32_t *tets,
+                      const int32_t *sdofs, int64_t e0, int64_t e1, const double *c_stiff,
+                      const double *c_mass, int rule_stiff, int rule_mass,
+                      const int32_t *rowptr, const int32_t *colidx, double *vals) {
+  const int n = order == 1 ? 4 : 10;
+  double P[12], mat[100];
+  int64_t missing = 0;
+  for (int64_t e = e0; e < e1; ++e) {
+    for (int v = 0; v < 4; ++v)
+      for (int d = 0; d < 3; ++d) P[3 * v + d] = xyz[3 * (tets[4 * e + v] - 1) + d];
+    double cs = c_stiff ? c_stiff[e] : 1.0, cm = c_mass ? c_mass[e] : 1.0;
+    if (or_element_matrix(ops, order, P, cs, cm, rule_stiff, rule_mass, mat)) return -1;
+    for (int j = 0; j < n; ++j) { /* test = row */
+      const int32_t row = sdofs[n * e + j];
+      if (row < 0) continue;
+      for (int i = 0; i < n; ++i) { /* field = column */
+        const int32_t col = sdofs[n * e + i];
+        if (col < 0) continue;
+        int32_t lo = rowptr[row], hi = rowptr[row + 1] - 1;
+        while (lo < hi) {
+          const int32_t mid = (lo + hi) >> 1;
+          if (colidx[mid] < col)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        if (lo > hi || colidx[lo] != col) {
+          ++missing;
+          continue;
+        }
+        vals[lo] += mat[i * n + j];
+      }
+    }
+  }
+  return missing;
+}
+
 int or_assemble(int order, int ops, int64_t nV, const double *xyz, int64_t nT,
                 const int32_t *tets, const double *c_stiff, const double *c_mass,
                 int rule_stiff, int rule_mass, int64_t *nrows, int64_t *nnz,
@@ -528,6 +575,96 @@ static double dot(int64_t n, const double *a, const double *b) {
   double s = 0.0;
   for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
   return s;
+}
+
+/* The iteration of or_pcg below, run for a fixed number of iterations with */
+/* its SpMV and vector updates split over `threads` host threads by row      */
+/* ranges (the shared-memory counterpart of the reference's MPI row          */
+/* partition); dot products are summed per thread, then in thread order.     */
+/* The CPU baseline of the solve: returns the wall time per iteration in     */
+/* seconds (setup excluded), x holds the iterate, *relres its explicit       */
+/* relative residual.                                                        */
+double or_pcg_fixed(int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *vals,
+                    const double *b, double *x, int iters, int threads, double *relres) {
+  double *dinv = (double *)malloc(sizeof(double) * (size_t)n);
+  double *r = (double *)malloc(sizeof(double) * (size_t)n);
+  double *z = (double *)malloc(sizeof(double) * (size_t)n);
+  double *p = (double *)malloc(sizeof(double) * (size_t)n);
+  double *q = (double *)malloc(sizeof(double) * (size_t)n);
+  double part[3 * 256];
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  double rz = 0.0, t0 = 0.0, t1 = 0.0;
+#pragma omp parallel num_threads(threads)
+  {
+    const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+    const int64_t i0 = n * t / nt, i1 = n * (t + 1) / nt;
+    for (int64_t i = i0; i < i1; ++i) {
+      dinv[i] = 0.0;
+      for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+        if (colidx[k] == i) dinv[i] = 1.0 / vals[k];
+      x[i] = 0.0;
+      r[i] = b[i];
+      z[i] = p[i] = dinv[i] * r[i];
+    }
+    double s = 0.0;
+    for (int64_t i = i0; i < i1; ++i) s += r[i] * z[i];
+    part[t] = s;
+#pragma omp barrier
+#pragma omp single
+    {
+      rz = 0.0;
+      for (int k = 0; k < nt; ++k) rz += part[k];
+      t0 = omp_get_wtime();
+    }
+    for (int it = 0; it < iters; ++it) {
+      double pq = 0.0;
+      for (int64_t i = i0; i < i1; ++i) {
+        double acc = 0.0;
+        for (int32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) acc += vals[k] * p[colidx[k]];
+        q[i] = acc;
+        pq += p[i] * acc;
+      }
+      part[t] = pq;
+#pragma omp barrier
+      double pqs = 0.0;
+      for (int k = 0; k < nt; ++k) pqs += part[k];
+      const double alpha = rz / pqs;
+      double rz2 = 0.0;
+      for (int64_t i = i0; i < i1; ++i) {
+        x[i] += alpha * p[i];
+        r[i] -= alpha * q[i];
+        z[i] = dinv[i] * r[i];
+        rz2 += r[i] * z[i];
+      }
+#pragma omp barrier
+      part[256 + t] = rz2;
+#pragma omp barrier
+      double rzs = 0.0;
+      for (int k = 0; k < nt; ++k) rzs += part[256 + k];
+      const double beta = rzs / rz;
+      for (int64_t i = i0; i < i1; ++i) p[i] = z[i] + beta * p[i];
+#pragma omp barrier
+#pragma omp single
+      rz = rzs;
+    }
+#pragma omp single
+    t1 = omp_get_wtime();
+  }
+  or_spmv(n, rowptr, colidx, vals, x, q);
+  double rr = 0.0, bb = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    const double d = b[i] - q[i];
+    rr += d * d;
+    bb += b[i] * b[i];
+  }
+  *relres = sqrt(rr) / (bb > 0 ? sqrt(bb) : 1.0);
+  free(dinv);
+  free(r);
+  free(z);
+  free(p);
+  free(q);
+  return iters > 0 ? (t1 - t0) / iters : 0.0;
 }
 
 /* Jacobi-preconditioned CG (block size 1), the algorithm behind              */

@@ -45,6 +45,10 @@ int or_assemble(int order, int ops, int64_t nV, const double *xyz, int64_t nT,
                 const int32_t *tets, const double *c_stiff, const double *c_mass,
                 int rule_stiff, int rule_mass, int64_t *nrows, int64_t *nnz,
                 int32_t **rowptr, int32_t **colidx, double **vals);
+int64_t or_accumulate(int ops, int order, const double *xyz, const int32_t *tets,
+                      const int32_t *sdofs, int64_t e0, int64_t e1, const double *c_stiff,
+                      const double *c_mass, int rule_stiff, int rule_mass,
+                      const int32_t *rowptr, const int32_t *colidx, double *vals);
 int or_load_vector(int order, int64_t nV, const double *xyz, int64_t nT, const int32_t *tets,
                    int kind, const double *f, int rule, double *F);
 int or_restrict(int64_t nrows, const int32_t *rowptr, const int32_t *colidx,
@@ -54,6 +58,8 @@ void or_free(void *p);
 
 void or_spmv(int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *vals,
              const double *x, double *y);
+double or_pcg_fixed(int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *vals,
+                    const double *b, double *x, int iters, int threads, double *relres);
 int or_pcg(int64_t n, const int32_t *rowptr, const int32_t *colidx, const double *vals,
            const double *b, double *x, double tol, int maxit, int *iters, double *relres);
 

@@ -79,6 +79,10 @@ def _declare(L):
     L.or_dd_grad.argtypes = [_I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                              _P]
     L.or_adjoint_grad_k.argtypes = [_I, _I64, _P, _I64, _P, _P, _I, _P, _P, _P]
+    L.or_accumulate.argtypes = [_I, _I, _P, _P, _P, _I64, _I64, _P, _P, _I, _I, _P, _P, _P]
+    L.or_accumulate.restype = _I64
+    L.or_pcg_fixed.argtypes = [_I64, _P, _P, _P, _P, _P, _I, _I, C.POINTER(C.c_double)]
+    L.or_pcg_fixed.restype = C.c_double
 
 
 def _p(a):
@@ -288,3 +292,50 @@ def adjoint_grad_k(order: int, mesh, dofs, u, lam, rule=4):
     lib().or_adjoint_grad_k(order, mesh.nV, _p(xyz), mesh.nT, _p(tets), _p(d), rule, _p(u),
                             _p(lam), _p(g))
     return g
+
+
+def accumulate(order: int, mesh, ops: int, sdofs, rowptr, colidx, c_stiff=None, c_mass=None,
+               rule_stiff=4, rule_mass=4, threads=8):
+    """Element matrices of the whole mesh added into a given CSR pattern
+    (or_accumulate), on `threads` host threads that each take a contiguous
+    block of elements into their own value array; the blocks are summed in
+    block order.  sdofs: (nT, n) system index per element basis, -1 where
+    the dof is not in the system.  Returns (vals, number of element pairs
+    missing from the pattern)."""
+    import concurrent.futures as cf
+    xyz, tets = _c(mesh.xyz, np.float64), _c(mesh.tets, np.int32)
+    sd = _c(sdofs, np.int32)
+    rp, ci = _c(rowptr, np.int32), _c(colidx, np.int32)
+    cs = _c(c_stiff, np.float64) if c_stiff is not None else None
+    cm = _c(c_mass, np.float64) if c_mass is not None else None
+    nT = mesh.nT
+    threads = max(1, min(threads, nT))
+    cuts = [nT * t // threads for t in range(threads + 1)]
+    L = lib()
+
+    def run(t):
+        v = np.zeros(ci.size)
+        miss = L.or_accumulate(ops, order, _p(xyz), _p(tets), _p(sd), cuts[t], cuts[t + 1],
+                               _p(cs), _p(cm), rule_stiff, rule_mass, _p(rp), _p(ci), _p(v))
+        if miss < 0:
+            raise ValueError("bad rule")
+        return v, miss
+
+    with cf.ThreadPoolExecutor(threads) as ex:
+        parts = list(ex.map(run, range(threads)))
+    vals = parts[0][0]
+    for v, _ in parts[1:]:
+        vals += v
+    return vals, sum(m for _, m in parts)
+
+
+def pcg_fixed(rowptr, colidx, vals, b, iters, threads=1):
+    """`iters` Jacobi-PCG iterations from x = 0 on `threads` host threads
+    (or_pcg_fixed); returns (x, seconds per iteration, explicit relres)."""
+    rp, ci, va = _c(rowptr, np.int32), _c(colidx, np.int32), _c(vals, np.float64)
+    b = _c(b, np.float64)
+    x = np.zeros_like(b)
+    rel = C.c_double()
+    t = lib().or_pcg_fixed(b.size, _p(rp), _p(ci), _p(va), _p(b), _p(x), iters, threads,
+                           C.byref(rel))
+    return x, t, rel.value

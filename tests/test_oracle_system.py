@@ -335,3 +335,19 @@ def test_adjoint_element_gradient_against_assembled_operator(order, rule):
         r, c, v = O.assemble(order, m, O.OP_STIFF, c_stiff=ki, rule_stiff=rule)
         Ai = sp.csr_matrix((v, c, r), shape=(N, N))
         assert abs(g[i] + lam @ (Ai @ u)) <= 1e-12 * max(1.0, abs(g[i]))
+
+
+def test_threaded_fixed_iteration_pcg_is_the_oracle_pcg():
+    # the CPU baseline of the solve: one thread repeats or_pcg bit for bit;
+    # several threads change only the summation order of the dot products
+    p = W.config_problem(2, n=6)
+    m = p.mesh
+    rp, ci, va = O.assemble(2, m, O.OP_STIFF, c_stiff=p.k, rule_stiff=4)
+    fx = O.dirichlet_mask(2, m, list(p.dirichlet_geo))
+    rp, ci, va = O.restrict(rp, ci, va, fx)
+    b = p.f[fx == 0]
+    x0, info = O.pcg(rp, ci, va, b, tol=1e-10)
+    x1, t1, rel1 = O.pcg_fixed(rp, ci, va, b, info["iters"], threads=1)
+    assert np.array_equal(x1, x0) and rel1 == info["relres"] and t1 > 0
+    x3, _, rel3 = O.pcg_fixed(rp, ci, va, b, info["iters"], threads=3)
+    assert rel3 < 1e-9 and np.abs(x3 - x0).max() <= 1e-9 * np.abs(x0).max()
