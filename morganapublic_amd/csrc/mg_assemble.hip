@@ -281,6 +281,32 @@ struct RecView<RecP1, LROW> {
     return RecView{q.x, (uint32_t)q.y};
   }
 };
+// wide records: uint16 positions
+template <int LROW>
+struct RecView<RecP2w, LROW> {
+  static constexpr int kElim = 0xFFFF;
+  int32_t el;
+  uint32_t w[5];
+  __device__ __forceinline__ int pos(int m) const { return (w[m >> 1] >> (16 * (m & 1))) & 0xFFFF; }
+  __device__ __forceinline__ static RecView load(const RecP2w *__restrict__ rec, int j, int) {
+    const int2 *q = reinterpret_cast<const int2 *>(rec) + 3 * (size_t)j;  // 24 bytes, 8-aligned
+    const int2 q0 = q[0], q1 = q[1], q2 = q[2];
+    return RecView{q0.x, {(uint32_t)q0.y, (uint32_t)q1.x, (uint32_t)q1.y, (uint32_t)q2.x,
+                          (uint32_t)q2.y}};
+  }
+};
+template <int LROW>
+struct RecView<RecP1w, LROW> {
+  static constexpr int kElim = 0xFFFF;
+  int32_t el;
+  uint32_t w[2];
+  __device__ __forceinline__ int pos(int m) const { return (w[m >> 1] >> (16 * (m & 1))) & 0xFFFF; }
+  __device__ __forceinline__ static RecView load(const RecP1w *__restrict__ rec, int j, int) {
+    const RecP1w q = rec[j];  // one dwordx3 load
+    return RecView{q.el, {(uint32_t)q.pos[0] | ((uint32_t)q.pos[1] << 16),
+                          (uint32_t)q.pos[2] | ((uint32_t)q.pos[3] << 16)}};
+  }
+};
 // compact P2: the row's own column (basis LROW) is its diagonal, `self`
 template <int LROW>
 struct RecP2cView {
@@ -411,15 +437,19 @@ __device__ __forceinline__ void row_values(const T e[6], T md, T v[NL]) {
 // incidence j is added, the edge values of j+1 and the record of j+3 are in
 // flight; edge rows (about 5 incidences) and FP32 rows load theirs in chunks
 // instead.
+//
+// GACC (blocks whose entries exceed the LDS budget; LPR = 1): each lane
+// zeroes and accumulates its own row in the values array itself, in the same
+// incidence order, so the result is the staged form's bit for bit.
 template <int NL, int LROW, typename T, bool MASS, bool EXACT, bool ELIM, int ROWS, int LPR,
-          typename Rec>
+          typename Rec, bool GACC = false>
 __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
     int32_t row_begin, int32_t row_end, int32_t dummy, EdgePack ep,
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ inc_ptr,
     const int32_t *__restrict__ diagpos, const Rec *__restrict__ rec, const T *__restrict__ geo,
     ElemSrc es, T *__restrict__ vals) {
+  static_assert(!GACC || LPR == 1, "global accumulation takes one lane per row");
   extern __shared__ __align__(16) char lds_raw[];
-  T *acc = reinterpret_cast<T *>(lds_raw);
   constexpr int NT = ROWS * LPR;
   constexpr int GS = geo_stride<T, MASS>();
   const int lane = threadIdx.x;
@@ -428,6 +458,7 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
   const int r1 = min(r0 + ROWS, row_end);
   const int s0 = rowptr[r0];
   const int ns = rowptr[r1] - s0;
+  T *acc = GACC ? vals + s0 : reinterpret_cast<T *>(lds_raw);
   // Edge rows and FP32 rows load their incidences in chunks (below) and
   // issue the lane's row pointers before the LDS is cleared, needing them
   // together after it (one round trip; otherwise the row offset is loaded
@@ -452,7 +483,13 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
     i = inc_ptr[rr] + sub;
     if (kSelf) self = diagpos[rr] - s0 - base;  // position of the diagonal in the row
   }
-  for (int s = lane; s < ns; s += NT) acc[s] = (T)0;
+  if constexpr (GACC) {
+    // each lane clears its own row (it is the only one to touch it)
+    if (r < r1)
+      for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) vals[k] = (T)0;
+  } else {
+    for (int s = lane; s < ns; s += NT) acc[s] = (T)0;
+  }
   __syncthreads();
   if (kChunked) asm volatile("" ::"v"(base), "v"(iend), "v"(i), "v"(self));
   if (r < r1) {
@@ -533,6 +570,16 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
       T v[NL];
       row_values<NL, LROW, T, MASS, EXACT>(A.e, A.md, v);
       const bool valid = j < iend;
+      if constexpr (GACC) {
+        if (valid) {
+#pragma unroll
+          for (int m = 0; m < NL; ++m) {
+            const int pm = A.r.pos(m);
+            if (!ELIM || pm != RV::kElim) acc[base + pm] += v[m];
+          }
+        }
+        return;
+      }
       const int bsel = valid ? base : dummy;
       int idx[NL];
 #pragma unroll
@@ -649,6 +696,7 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
       }
     }
   }
+  if constexpr (GACC) return;
   __syncthreads();
   // values are written once and not read again by this assembly: streaming
   // (nontemporal) stores keep them from evicting element records from L2
@@ -698,39 +746,45 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
   const Rec *rec = (const Rec *)S.inc_rec;
   for (const RowSegment &sg : S.segs) {
     const int nblk = (sg.row_end - sg.row_begin + sg.rows - 1) / sg.rows;
-    const size_t lds = sizeof(T) * (size_t)(sg.lds_slots + 256);
-    auto go = [&](auto lr, auto ms, auto ex, auto rw, auto lp) {
+    const size_t lds = sg.gacc ? 0 : sizeof(T) * (size_t)(sg.lds_slots + 256);
+    auto go = [&](auto lr, auto ms, auto ex, auto rw, auto lp, auto ga) {
       if (elim)
         k_asm_rows<NL, decltype(lr)::value, T, decltype(ms)::value, decltype(ex)::value, true,
-                   decltype(rw)::value, decltype(lp)::value, Rec>
+                   decltype(rw)::value, decltype(lp)::value, Rec, decltype(ga)::value>
             <<<nblk, decltype(rw)::value * decltype(lp)::value, lds, c->stream>>>(
                 sg.row_begin, sg.row_end, sg.lds_slots, ep, S.rowptr, S.inc_ptr, S.diagpos, rec, geo,
                 es, vals);
       else
         k_asm_rows<NL, decltype(lr)::value, T, decltype(ms)::value, decltype(ex)::value, false,
-                   decltype(rw)::value, decltype(lp)::value, Rec>
+                   decltype(rw)::value, decltype(lp)::value, Rec, decltype(ga)::value>
           <<<nblk, decltype(rw)::value * decltype(lp)::value, lds, c->stream>>>(
               sg.row_begin, sg.row_end, sg.lds_slots, ep, S.rowptr, S.inc_ptr, S.diagpos, rec, geo,
                 es, vals);
     };
     // vertex-dof rows: 64 rows per workgroup, 1/2/4 lanes per row; P2 edge-dof
-    // rows: 64/128/256 rows per workgroup, 1/2 lanes per row
+    // rows: 64/128/256 rows per workgroup, 1/2 lanes per row; blocks past the
+    // LDS budget: 64 rows, one lane per row, accumulated in place
     auto shape = [&](auto lr, auto ms, auto ex) {
+      if (sg.gacc) {
+        go(lr, ms, ex, IC<64>{}, IC<1>{}, BC<true>{});
+        return;
+      }
+      const BC<false> st{};
       if constexpr (decltype(lr)::value == 0) {
         const int lpr = sizeof(T) == 4 ? sg.lpr32 : sg.lpr;
         if (lpr == 4)
-          go(lr, ms, ex, IC<64>{}, IC<4>{});
+          go(lr, ms, ex, IC<64>{}, IC<4>{}, st);
         else if (lpr == 2)
-          go(lr, ms, ex, IC<64>{}, IC<2>{});
+          go(lr, ms, ex, IC<64>{}, IC<2>{}, st);
         else
-          go(lr, ms, ex, IC<64>{}, IC<1>{});
+          go(lr, ms, ex, IC<64>{}, IC<1>{}, st);
       } else {
         if (sg.rows == 256)
-          sg.lpr == 2 ? go(lr, ms, ex, IC<256>{}, IC<2>{}) : go(lr, ms, ex, IC<256>{}, IC<1>{});
+          sg.lpr == 2 ? go(lr, ms, ex, IC<256>{}, IC<2>{}, st) : go(lr, ms, ex, IC<256>{}, IC<1>{}, st);
         else if (sg.rows == 128)
-          sg.lpr == 2 ? go(lr, ms, ex, IC<128>{}, IC<2>{}) : go(lr, ms, ex, IC<128>{}, IC<1>{});
+          sg.lpr == 2 ? go(lr, ms, ex, IC<128>{}, IC<2>{}, st) : go(lr, ms, ex, IC<128>{}, IC<1>{}, st);
         else
-          sg.lpr == 2 ? go(lr, ms, ex, IC<64>{}, IC<2>{}) : go(lr, ms, ex, IC<64>{}, IC<1>{});
+          sg.lpr == 2 ? go(lr, ms, ex, IC<64>{}, IC<2>{}, st) : go(lr, ms, ex, IC<64>{}, IC<1>{}, st);
       }
     };
     auto terms = [&](auto lr) {
@@ -755,11 +809,15 @@ static void assemble_t(Ctx *c, int sys, const mg_operator *op, int prec) {
   const bool stiff = op->terms & MG_TERM_STIFF, mass = op->terms & MG_TERM_MASS;
   upload_tables(c, c->order, stiff ? op->stiff_rule : 0, mass ? op->mass_rule : 0, prec);
   T *vals = prec == MG_PREC_F64 ? (T *)S.vals64 : (T *)S.vals32;
-  if (c->nloc == 10)
-    S.rec_words == 3 ? launch_rows<10, T, RecP2c>(c, S, op, vals, sys == MG_SYS_INTERIOR)
-                     : launch_rows<10, T, RecP2>(c, S, op, vals, sys == MG_SYS_INTERIOR);
-  else
-    launch_rows<4, T, RecP1>(c, S, op, vals, sys == MG_SYS_INTERIOR);
+  const bool elim = sys == MG_SYS_INTERIOR;
+  switch (S.rec_kind) {
+    case kRecP2c: launch_rows<10, T, RecP2c>(c, S, op, vals, elim); break;
+    case kRecP2: launch_rows<10, T, RecP2>(c, S, op, vals, elim); break;
+    case kRecP2w: launch_rows<10, T, RecP2w>(c, S, op, vals, elim); break;
+    case kRecP1: launch_rows<4, T, RecP1>(c, S, op, vals, elim); break;
+    case kRecP1w: launch_rows<4, T, RecP1w>(c, S, op, vals, elim); break;
+    default: MG_CHECK(false, MG_ERR_STATE, "assemble: no incidence records");
+  }
 }
 
 // ---- adjoint element kernel ---------------------------------------------------

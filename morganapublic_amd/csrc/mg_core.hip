@@ -22,7 +22,8 @@ void dfree(Ctx *c, void *p) {
 void free_system(Ctx *c, System &S) {
   for (void *p : {(void *)S.rowptr, (void *)S.colidx, (void *)S.diagpos, (void *)S.slot,
                   (void *)S.inc_ptr, S.inc_rec, (void *)S.vals64, (void *)S.vals32,
-                  (void *)S.dinv64, (void *)S.dinv32, (void *)S.sdesc, (void *)S.cofs})
+                  (void *)S.dinv64, (void *)S.dinv32, (void *)S.sdesc, (void *)S.cofs,
+                  (void *)S.long_rows})
     dfree(c, p);
   S = System();
 }

@@ -35,6 +35,18 @@ struct alignas(4) RecP2c {
   uint32_t lo, hi;  // 63 bits of positions
 };
 
+// Wide records for rows past 254 entries: uint16 positions (0xFFFF:
+// eliminated column), 24 bytes (P2) / 12 bytes (P1).
+struct alignas(8) RecP2w {
+  int32_t el;
+  uint16_t pos[10];
+};
+struct alignas(4) RecP1w {
+  int32_t el;
+  uint16_t pos[4];
+};
+enum RecKind { kRecP1 = 1, kRecP1w, kRecP2, kRecP2c, kRecP2w };
+
 // Rows of one assembly launch: blocks of `rows` rows, each staging its CSR
 // entries in LDS; lrow is the local basis every incidence of these rows is relabelled to
 // (0 for vertex-dof rows, 4 for P2 edge-dof rows).
@@ -42,6 +54,9 @@ struct RowSegment {
   int32_t row_begin, row_end, lds_slots, lrow;
   int32_t rows, lpr;  // rows per workgroup, lanes per row
   int32_t lpr32;      // lanes per row of FP32 assemblies
+  // 1: the block's entries exceed the LDS budget and are accumulated in the
+  // values array itself, one lane per row (no staging)
+  int32_t gacc;
 };
 
 constexpr int kSpmvRows = 64;  // rows per SpMV block (128 for short rows)
@@ -85,11 +100,16 @@ struct System {
   int32_t *slot = nullptr;     // nT * nloc * nloc element -> CSR slot map
   int32_t *inc_ptr = nullptr;  // nrows + 1 (row -> incidences)
   void *inc_rec = nullptr;     // n_inc records (RecP1, RecP2 or RecP2c), sorted by (row, l, e)
-  int rec_words = 0;           // int32 words per record: 2 (RecP1), 4 (RecP2), 3 (RecP2c)
+  int rec_words = 0;           // int32 words per record: 2 (RecP1), 3 (RecP1w, RecP2c), 4 (RecP2), 6 (RecP2w)
+  int rec_kind = 0;            // RecKind
   std::vector<RowSegment> segs;
   // SpMV row blocks: each holds <= kSpmvNnz entries and <= kSpmvRows rows
   BlkDesc *sdesc = nullptr;
   int32_t n_sblk = 0;
+  // rows longer than one block (kSpmvNnz entries) are in no block: a
+  // workgroup per row takes them (k_long_rows)
+  int32_t *long_rows = nullptr;
+  int32_t n_long = 0;
   int srl = 4;  // lanes per row summing a block: 4 (64-row blocks) or 2 (128-row blocks)
   // the columns in 16-bit form (c16: every block fits kColClusters clusters,
   // bases in sdesc), read by the SpMV streams instead of colidx

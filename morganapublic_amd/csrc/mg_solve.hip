@@ -321,6 +321,59 @@ __global__ __launch_bounds__(kThreads) void k_resid64_stream(
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// Rows longer than an SpMV block belong to no block: one workgroup takes one
+// such row at a time and its lanes sum the row's entries lane-strided, then
+// in a fixed tree over lanes and waves (deterministic).  MODE 0: y = A x;
+// 1: q = A p with the p.q partials; 2: r = b - A x with the r.r partials.
+// Partials go to part[blockIdx.x] (the caller passes the slots after the
+// stream kernel's).
+template <typename T>
+__device__ __forceinline__ T block_sum_t(T v, T *sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  __syncthreads();
+  if (ln == 0) sh[w] = v;
+  __syncthreads();
+  T t = (T)0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
+  __syncthreads();
+  return t;
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kThreads) void k_long_rows(
+    int nl, const int32_t *__restrict__ long_rows, const int32_t *__restrict__ rowptr,
+    const int32_t *__restrict__ colidx, const T *__restrict__ vals, const T *__restrict__ x,
+    const T *__restrict__ b, T *__restrict__ y, double *__restrict__ part,
+    const PcgState *__restrict__ st) {
+  __shared__ T shv[kThreads / 64];
+  __shared__ double sh[kThreads / 64];
+  if (MODE == 1 && st->done) return;
+  double acc = 0.0;
+  for (int i = blockIdx.x; i < nl; i += gridDim.x) {
+    const int row = long_rows[i];
+    T s = (T)0;
+    for (int k = rowptr[row] + (int)threadIdx.x; k < rowptr[row + 1]; k += kThreads)
+      s += vals[k] * x[colidx[k]];
+    s = block_sum_t(s, shv);
+    if (threadIdx.x == 0) {
+      if (MODE == 2) {
+        const double ri = (double)b[row] - (double)s;
+        y[row] = (T)ri;
+        acc += ri * ri;
+      } else {
+        y[row] = s;
+        if (MODE == 1) acc += (double)(s * x[row]);
+      }
+    }
+  }
+  if (MODE != 0) {
+    const double t = block_sum_all(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+  }
+}
+
 // The elementwise passes read 16-byte vectors (VEC: every array 16-byte
 // aligned), 8 entries per array per thread and round, all loads of a round
 // issued before any use; the n % (16 / sizeof(T)) last entries go to the
@@ -677,9 +730,16 @@ void ensure_dinv(Ctx *c, int sys, int prec) {
   MG_HIP(hipGetLastError());
 }
 
+// workgroups of k_long_rows (0: no long rows)
+static int grid_long(const System &S) { return S.n_long < 64 ? S.n_long : 64; }
+
+// workgroups of the stream kernels; their partials and the long-row kernel's
+// (npart in all) fit the kMaxBlocks-entry partial arrays
 static int grid_blocks(const System &S) {
-  return S.n_sblk < kMaxBlocks ? (S.n_sblk < 1 ? 1 : S.n_sblk) : kMaxBlocks;
+  const int cap = kMaxBlocks - grid_long(S);
+  return S.n_sblk < cap ? (S.n_sblk < 1 ? 1 : S.n_sblk) : cap;
 }
+static int npart(const System &S) { return grid_blocks(S) + grid_long(S); }
 
 // Calls f(C16, RL) with the system's column form and lanes per row as
 // compile-time constants.
@@ -702,10 +762,13 @@ static void spmv_t(Ctx *c, System &S, const T *vals, const T *x, T *y) {
         <<<grid_blocks(S), kThreads, 0, c->stream>>>(S.n_sblk, S.sdesc, S.rowptr, S.colidx, S.cofs,
                                                      vals, x, y);
   });
+  if (S.n_long)
+    k_long_rows<T, 0><<<grid_long(S), kThreads, 0, c->stream>>>(
+        S.n_long, S.long_rows, S.rowptr, S.colidx, vals, x, nullptr, y, nullptr, nullptr);
   MG_HIP(hipGetLastError());
 }
 
-// q = A p with the p.q partials, in the system's column form
+// q = A p with the p.q partials (npart(S) of them), in the system's column form
 template <typename T>
 static void spmv_pq(Ctx *c, const System &S, int gb, const T *vals, const T *p, T *q,
                     double *pq_part, const PcgState *st) {
@@ -713,6 +776,9 @@ static void spmv_pq(Ctx *c, const System &S, int gb, const T *vals, const T *p, 
     k_spmv_pq_stream<T, decltype(cf)::value, decltype(rl)::value><<<gb, kThreads, 0, c->stream>>>(
         S.n_sblk, S.sdesc, S.rowptr, S.colidx, S.cofs, vals, p, q, pq_part, st);
   });
+  if (S.n_long)
+    k_long_rows<T, 1><<<grid_long(S), kThreads, 0, c->stream>>>(
+        S.n_long, S.long_rows, S.rowptr, S.colidx, vals, p, nullptr, q, pq_part + gb, st);
 }
 
 void spmv_launch(Ctx *c, int sys, int prec, const void *x, void *y) {
@@ -855,7 +921,7 @@ static PcgState pcg_core(Ctx *c, System &S, const T *vals, const T *dinv, const 
       double *rz_prev = (it & 1) ? P.rz0 : P.rz1;
       double *rz_new = (it & 1) ? P.rz1 : P.rz0;
       spmv_pq<T>(c, S, gb, vals, p, q, P.pq, w.st);
-      vec_update<T>(c, gv, n, x, r, p, q, dinv, P.pq, gb, rz_prev, gv,
+      vec_update<T>(c, gv, n, x, r, p, q, dinv, P.pq, npart(S), rz_prev, gv,
                                                   rz_new, P.rr, w.st);
       vec_pupdate<T>(c, gv, n, r, p, dinv, rz_new, rz_prev, P.rr, gv, it,
                                                    w.st);
@@ -938,7 +1004,7 @@ static int pcg_reliable(Ctx *c, System &S, const double *b, double *x, double bn
       double *rz_new = (it & 1) ? P.rz1 : P.rz0;
       spmv_pq<float>(c, S, gb, S.vals32, w.p32, w.q32, P.pq, w.st);
       vec_update<float>(c, gv, n, w.x32, w.r32, w.p32, w.q32, S.dinv32,
-                                                      P.pq, gb, rz_prev, gv, rz_new, P.rr, w.st);
+                                                      P.pq, npart(S), rz_prev, gv, rz_new, P.rr, w.st);
       vec_pupdate<float>(c, gv, n, w.r32, w.p32, S.dinv32, rz_new, rz_prev,
                                                        P.rr, gv, it, w.st);
     }
@@ -994,7 +1060,11 @@ static double true_relres(Ctx *c, System &S, const double *b, const double *x, W
     k_resid64_stream<decltype(cf)::value, decltype(rl)::value><<<g, kThreads, 0, c->stream>>>(
         S.n_sblk, S.sdesc, S.rowptr, S.colidx, S.cofs, S.vals64, x, b, w.rr64, w.part.tmp);
   });
-  k_finalize_sum<<<1, kThreads, 0, c->stream>>>(w.part.tmp, g, &w.st->rr);
+  if (S.n_long)
+    k_long_rows<double, 2><<<grid_long(S), kThreads, 0, c->stream>>>(
+        S.n_long, S.long_rows, S.rowptr, S.colidx, S.vals64, x, b, w.rr64, w.part.tmp + g,
+        nullptr);
+  k_finalize_sum<<<1, kThreads, 0, c->stream>>>(w.part.tmp, npart(S), &w.st->rr);
   MG_HIP(hipGetLastError());
   PcgState h = read_state(c, w.st);
   return std::sqrt(h.rr) / bnorm;

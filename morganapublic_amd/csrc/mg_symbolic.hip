@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "mg_internal.h"
 
@@ -361,11 +362,15 @@ __global__ __launch_bounds__(256) void k_col_compress(BlkDesc *__restrict__ desc
   if (any) *fail = 1;
 }
 
+// Records with uint8 (RecP1, RecP2) or uint16 (RecP1w, RecP2w) positions; the
+// largest value of the position type marks an eliminated column.
 template <int NL, typename Rec>
 __global__ void k_inc_records(int64_t n_inc, const uint64_t *__restrict__ keys,
                               const int32_t *__restrict__ vals, const int32_t *__restrict__ slot,
                               const int32_t *__restrict__ rowptr, Rec *__restrict__ rec,
                               int *__restrict__ bad) {
+  using P = typename std::remove_reference<decltype(rec->pos[0])>::type;
+  constexpr int kElim = (int)(P)~0u;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_inc) return;
   int32_t v = vals[i];
@@ -378,12 +383,12 @@ __global__ void k_inc_records(int64_t n_inc, const uint64_t *__restrict__ keys,
   for (int m = 0; m < NL; ++m) {
     // column of relabelled basis m = original basis c_sigma[l][m]; row basis l
     int32_t s = slot[e * NL * NL + c_sigma[l][m] * NL + l];
-    int p = s < 0 ? 0xFF : s - base;
-    if (s >= 0 && (p < 0 || p >= 0xFF)) {
+    int p = s < 0 ? kElim : s - base;
+    if (s >= 0 && (p < 0 || p >= kElim)) {
       atomicAdd(bad, 1);
-      p = 0xFF;
+      p = kElim;
     }
-    r.pos[m] = (uint8_t)p;
+    r.pos[m] = (P)p;
   }
   rec[i] = r;
 }
@@ -431,20 +436,42 @@ static int env_int(const char *name, int dflt) {
   return v ? atoi(v) : dflt;
 }
 
+// LDS budget of a staged row block: its entries plus the 256-slot region
+// that takes eliminated columns, in FP64, within 64 KiB
+static bool block_fits(int entries) { return (entries + 256) * 8 <= 64 * 1024; }
+
 static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvrows, int nl) {
   S.segs.clear();
+  auto push = [&](int64_t a, int64_t b, int lrow, int rows, int lpr, int lpr32, bool gacc) {
+    if (b <= a) return;
+    int mx = 0;
+    for (int64_t r = a; r < b; r += rows) mx = std::max(mx, rp[std::min<int64_t>(r + rows, b)] - rp[r]);
+    S.segs.push_back(RowSegment{(int32_t)a, (int32_t)b, gacc ? 0 : mx, lrow, rows,
+                                gacc ? 1 : lpr, gacc ? 1 : lpr32, gacc ? 1 : 0});
+  };
   auto add = [&](int64_t a, int64_t b, int lrow, int rows, int lpr, int lpr32) {
     if (b <= a) return;
+    // halve the rows per block until every block fits the LDS budget (64
+    // rows at the least); 64-row blocks that still do not fit go to
+    // segments that accumulate in global memory instead
     for (;;) {
       int mx = 0;
       for (int64_t r = a; r < b; r += rows)
         mx = std::max(mx, rp[std::min<int64_t>(r + rows, b)] - rp[r]);
-      if ((mx + 256) * 8 <= 64 * 1024 || rows == 64) {
-        S.segs.push_back(RowSegment{(int32_t)a, (int32_t)b, mx, lrow, rows, lpr, lpr32});
-        return;
-      }
-      rows /= 2;  // a block's entries must fit 64 KiB of LDS
+      if (block_fits(mx) || rows == 64) break;
+      rows /= 2;
     }
+    int64_t run = a;
+    bool run_g = !block_fits(rp[std::min<int64_t>(a + rows, b)] - rp[a]);
+    for (int64_t r = a; r < b; r += rows) {
+      const bool g = !block_fits(rp[std::min<int64_t>(r + rows, b)] - rp[r]);
+      if (g != run_g) {
+        push(run, r, lrow, rows, lpr, lpr32, run_g);
+        run = r;
+        run_g = g;
+      }
+    }
+    push(run, b, lrow, rows, lpr, lpr32, run_g);
   };
   auto pick = [](int v, int a, int b, int c) { return v == a || v == b || v == c; };
   // launch shapes (MGFE_VERTEX_LPR 1/2/4, MGFE_EDGE_ROWS 64/128/256,
@@ -540,30 +567,49 @@ void build_pattern(Ctx *c, int sys) {
   int maxnnz = 0;
   for (int64_t r = 0; r < nrows; ++r) maxnnz = std::max(maxnnz, rp[r + 1] - rp[r]);
   S.max_row_nnz = maxnnz;
-  // P2: 12-byte records when every row has <= 127 entries (MGFE_REC16=1
-  // keeps the 16-byte form)
+  // record form by the longest row: P2 12-byte records with 7-bit positions
+  // up to 127 entries (MGFE_REC16=1 keeps the 16-byte form), uint8
+  // positions up to 254, uint16 positions beyond (up to 65534)
+  MG_CHECK(maxnnz <= 65534, MG_ERR_SIZE, "pattern: a row has more than 65534 entries");
   const bool compact = nl == 10 && maxnnz <= 127 && env_int("MGFE_REC16", 0) == 0;
+  const bool wide = maxnnz > 254;
+  const int gi = blocks_for(S.n_inc, 256);
   if (compact) {
     S.inc_rec = dalloc<RecP2c>(c, S.n_inc);
     S.rec_words = 3;
-    k_inc_records_c<<<blocks_for(S.n_inc, 256), 256, 0, c->stream>>>(
-        S.n_inc, k1, v1, S.slot, S.rowptr, S.diagpos, (RecP2c *)S.inc_rec, bad);
-  } else if (nl == 10) {
+    S.rec_kind = kRecP2c;
+    k_inc_records_c<<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, S.slot, S.rowptr, S.diagpos,
+                                               (RecP2c *)S.inc_rec, bad);
+  } else if (nl == 10 && !wide) {
     S.inc_rec = dalloc<RecP2>(c, S.n_inc);
     S.rec_words = 4;
-    k_inc_records<10, RecP2><<<blocks_for(S.n_inc, 256), 256, 0, c->stream>>>(
-        S.n_inc, k1, v1, S.slot, S.rowptr, (RecP2 *)S.inc_rec, bad);
-  } else {
+    S.rec_kind = kRecP2;
+    k_inc_records<10, RecP2><<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, S.slot, S.rowptr,
+                                                        (RecP2 *)S.inc_rec, bad);
+  } else if (nl == 10) {
+    S.inc_rec = dalloc<RecP2w>(c, S.n_inc);
+    S.rec_words = 6;
+    S.rec_kind = kRecP2w;
+    k_inc_records<10, RecP2w><<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, S.slot, S.rowptr,
+                                                         (RecP2w *)S.inc_rec, bad);
+  } else if (!wide) {
     S.inc_rec = dalloc<RecP1>(c, S.n_inc);
     S.rec_words = 2;
-    k_inc_records<4, RecP1><<<blocks_for(S.n_inc, 256), 256, 0, c->stream>>>(
-        S.n_inc, k1, v1, S.slot, S.rowptr, (RecP1 *)S.inc_rec, bad);
+    S.rec_kind = kRecP1;
+    k_inc_records<4, RecP1><<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, S.slot, S.rowptr,
+                                                       (RecP1 *)S.inc_rec, bad);
+  } else {
+    S.inc_rec = dalloc<RecP1w>(c, S.n_inc);
+    S.rec_words = 3;
+    S.rec_kind = kRecP1w;
+    k_inc_records<4, RecP1w><<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, S.slot, S.rowptr,
+                                                        (RecP1w *)S.inc_rec, bad);
   }
   MG_HIP(hipGetLastError());
   int hb = read_back(c, bad);
   for (void *p : {(void *)k0, (void *)k1, (void *)v0, (void *)v1, (void *)cnt, (void *)bad})
     dfree(c, p);
-  MG_CHECK(hb == 0, MG_ERR_SIZE, "pattern: a row has more than 254 entries (uint8 positions)");
+  MG_CHECK(hb == 0, MG_ERR_STATE, "pattern: a record position does not fit its form");
   const int64_t nvrows =
       nl == 4 ? nrows : (sys == MG_SYS_FULL ? c->nV : c->nfree_vertex);
   make_segments(S, rp, nvrows, nl);
@@ -579,21 +625,36 @@ void build_pattern(Ctx *c, int sys) {
     const int want = env_int("MGFE_SPMV_ROWS", mean <= kSpmvNnz / (2 * kSpmvRows) ? 128 : 64);
     const int rows_cap = want == 128 ? 2 * kSpmvRows : kSpmvRows;
     S.srl = rows_cap == kSpmvRows ? 4 : 2;
+    // blocks of consecutive rows; a row longer than a block stands alone in
+    // the long-row list, and the blocks around it end before it and start
+    // after it
+    std::vector<int32_t> lr;
+    std::vector<std::pair<int32_t, int32_t>> br;
     while (r < nrows) {
+      if (rp[r + 1] - rp[r] > kSpmvNnz) {
+        lr.push_back((int32_t)r);
+        ++r;
+        continue;
+      }
       int64_t r1 = r + 1;
-      MG_CHECK(rp[r + 1] - rp[r] <= kSpmvNnz, MG_ERR_SIZE, "pattern: a row has more than 1024 entries (SpMV block)");
       while (r1 < nrows && r1 - r < rows_cap && rp[r1 + 1] - rp[r] <= kSpmvNnz) ++r1;
-      sb.push_back((int32_t)r1);
+      br.push_back({(int32_t)r, (int32_t)r1});
       r = r1;
     }
-    S.n_sblk = (int32_t)sb.size() - 1;
+    S.n_sblk = (int32_t)br.size();
     std::vector<BlkDesc> d(S.n_sblk);
     for (int32_t b = 0; b < S.n_sblk; ++b) {
       d[b] = BlkDesc{};
-      d[b].r0 = sb[b];
-      d[b].r1 = sb[b + 1];
-      d[b].s0 = rp[sb[b]];
-      d[b].ns = rp[sb[b + 1]] - rp[sb[b]];
+      d[b].r0 = br[b].first;
+      d[b].r1 = br[b].second;
+      d[b].s0 = rp[br[b].first];
+      d[b].ns = rp[br[b].second] - rp[br[b].first];
+    }
+    S.n_long = (int32_t)lr.size();
+    if (S.n_long) {
+      S.long_rows = dalloc<int32_t>(c, lr.size());
+      MG_HIP(hipMemcpyAsync(S.long_rows, lr.data(), sizeof(int32_t) * lr.size(),
+                            hipMemcpyHostToDevice, c->stream));
     }
     S.sdesc = dalloc<BlkDesc>(c, d.size());
     MG_HIP(hipMemcpyAsync(S.sdesc, d.data(), sizeof(BlkDesc) * d.size(), hipMemcpyHostToDevice,
@@ -603,7 +664,9 @@ void build_pattern(Ctx *c, int sys) {
   // 16-bit columns for the SpMV streams (MGFE_SPMV_COL32=1 keeps int32)
   if (S.n_sblk > 0 && env_int("MGFE_SPMV_COL32", 0) == 0) {
     S.cofs = dalloc<uint16_t>(c, S.nnz + 8);
-    MG_HIP(hipMemsetAsync(S.cofs + S.nnz, 0, 8 * sizeof(uint16_t), c->stream));
+    // zeroed: the entries of long rows (in no block) are never written, and
+    // the padding after the array is read past a block's end
+    MG_HIP(hipMemsetAsync(S.cofs, 0, (S.nnz + 8) * sizeof(uint16_t), c->stream));
     int *fail = dalloc<int>(c, 1);
     MG_HIP(hipMemsetAsync(fail, 0, sizeof(int), c->stream));
     k_col_compress<<<S.n_sblk, 256, 0, c->stream>>>(S.sdesc, S.colidx, S.cofs, (int)S.nrows,
@@ -616,9 +679,6 @@ void build_pattern(Ctx *c, int sys) {
       S.cofs = nullptr;
     }
   }
-  for (const RowSegment &sg : S.segs)
-    MG_CHECK((sg.lds_slots + 256) * 8 <= 64 * 1024, MG_ERR_SIZE,
-             "pattern: a 64-row block owns more than 7936 CSR entries");
   S.vals64 = dalloc<double>(c, S.nnz + 8);
   MG_HIP(hipMemsetAsync(S.vals64 + S.nnz, 0, 8 * sizeof(double), c->stream));
   S.vals32 = dalloc<float>(c, S.nnz + 8);

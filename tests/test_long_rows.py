@@ -1,12 +1,21 @@
 """Long CSR rows: a star of tetrahedra around one centre vertex.
 
 The centre's P2 row holds 1 + 2 V_s + E_s entries (its own vertex, the V_s
-surface vertices, the V_s centre edges and the E_s surface edges), so an
-icosphere surface gives rows past the 127 entries of the compact 12-byte
-incidence record (level 1: 205 entries; the 16-byte record with uint8
-positions takes over by itself) and past the 254 entries uint8 positions can
-address (level 2: 805 entries; the pattern build must refuse it with
-MG_ERR_SIZE).  P1 at level 2 gives a 163-entry row on the 8-byte P1 record.
+surface vertices, the V_s centre edges and the E_s surface edges), so
+icosphere surfaces of growing level walk the row through every form the
+library switches between by itself:
+  level 1 P2, 205 entries: past the 127 entries of the 12-byte record
+      (16-byte records, uint8 positions);
+  level 2 P2, 805 entries: past the 254 entries uint8 positions address
+      (24-byte records, uint16 positions);
+  level 3 P2, 3205 entries: past the 1024 entries of one SpMV row block
+      (the row is summed by the long-row kernel);
+  level 4 P2, 12805 entries: its 64-row block is past the LDS staging
+      budget (accumulated in the values array, one lane per row);
+  P1 at level 2 (163 entries) and level 4 (2563: 12-byte P1 records with
+      uint16 positions, a long SpMV row).
+traitsMatrixBuilder::buildEpetraCrs (traitsMatrixBuilder.hpp:400-440) has no
+row-length limit; neither has the library up to 65534 entries.
 """
 import numpy as np
 import pytest
@@ -59,7 +68,8 @@ def star_mesh(level):
 def test_star_mesh_rows_by_counting():
     """The oracle's pattern on the star meshes has the row lengths the
     counting above fixes, and the stiffness rows sum to zero."""
-    for level, order, want in ((1, 2, 205), (2, 2, 805), (2, 1, 163)):
+    for level, order, want in ((1, 2, 205), (2, 2, 805), (3, 2, 3205), (4, 2, 12805),
+                               (2, 1, 163), (4, 1, 2563)):
         m = star_mesh(level)
         vs, fs = icosphere(level)
         E_s = len({(min(a, b), max(a, b)) for f in fs for a, b in ((f[0], f[1]), (f[1], f[2]), (f[2], f[0]))})
@@ -81,7 +91,7 @@ def _check_values(got, rp, va, tol):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("level,order", [(1, 2), (2, 1)])
+@pytest.mark.parametrize("level,order", [(1, 2), (2, 2), (3, 2), (4, 2), (2, 1), (4, 1)])
 def test_long_rows_match_oracle(ctx, level, order):
     m = star_mesh(level)
     k = np.exp(W.stream_normal(0, W.STREAM_K, m.nT))
@@ -99,6 +109,11 @@ def test_long_rows_match_oracle(ctx, level, order):
         ctx.assemble(L.SYS_FULL, L.TERM_STIFF | L.TERM_MASS, stiff_coeff=L.COEFF_P0,
                      stiff_rule=4, mass_rule=15, prec=prec)
         _check_values(ctx.values(L.SYS_FULL, prec=prec).astype(np.float64), rp, va, tol)
+    # y = A x through the SpMV (long rows included) against the oracle's product
+    x = 2 * W.stream_uniform(0, W.STREAM_F, rp.size - 1) - 1
+    y = ctx.spmv(L.SYS_FULL, x)
+    yo = O.spmv(rp, ci, va, x)
+    assert np.abs(y - yo).max() <= 1e-12 * np.abs(yo).max()
     fixed = O.dirichlet_mask(order, m, [1])
     np.testing.assert_array_equal(ctx.dirichlet_mask(), fixed)
     rpi, cii, vai = O.restrict(rp, ci, va, fixed)
@@ -106,17 +121,29 @@ def test_long_rows_match_oracle(ctx, level, order):
                  stiff_rule=4, mass_rule=15)
     _check_values(ctx.values(L.SYS_INTERIOR), rpi, vai, 1e-12)
     b = 2 * W.stream_uniform(0, W.STREAM_F, rpi.size - 1) - 1
-    x, info = ctx.pcg(b, mode=L.SOLVE_F64, tol=1e-10)
-    assert info["status"] == L.MG_OK
     xo, _ = O.pcg(rpi, cii, vai, b, tol=1e-10)
-    assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+    for mode in (L.SOLVE_F64, L.SOLVE_F32_REFINE, L.SOLVE_F32_RELIABLE):
+        x, info = ctx.pcg(b, mode=mode, tol=1e-10)
+        assert info["status"] == L.MG_OK, (mode, info)
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
 
 
 @pytest.mark.gpu
-def test_row_past_uint8_positions_is_refused(ctx):
-    m = star_mesh(2)
-    ctx.mesh_set(m.xyz, m.tets, m.btri, m.bgeo)
-    ctx.dofmap_build(2)
-    with pytest.raises(L.MgfeError) as ei:
+@pytest.mark.parametrize("order", [1, 2])
+def test_long_rows_bit_identical_across_launch_shapes(ctx, monkeypatch, order):
+    # the accumulated-in-place blocks and the staged ones sum in the same
+    # incidence order: the level-4 star assembles to the same bits whatever
+    # lanes per row the staged blocks use
+    m = star_mesh(4)
+    k = np.exp(W.stream_normal(0, W.STREAM_K, m.nT))
+    out = []
+    for lpr in ("1", "4"):
+        monkeypatch.setenv("MGFE_VERTEX_LPR", lpr)
+        monkeypatch.setenv("MGFE_VERTEX_LPR32", lpr)
+        ctx.mesh_set(m.xyz, m.tets, m.btri, m.bgeo)
+        ctx.dofmap_build(order)
         ctx.pattern_build(L.SYS_FULL)
-    assert ei.value.code == L.MG_ERR_SIZE
+        ctx.field_set(L.FIELD_STIFF, k)
+        ctx.assemble(L.SYS_FULL, L.TERM_STIFF, stiff_coeff=L.COEFF_P0, stiff_rule=4)
+        out.append(ctx.values(L.SYS_FULL))
+    assert np.array_equal(out[0], out[1])
