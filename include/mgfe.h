@@ -149,15 +149,18 @@ int mg_dirichlet_mask_get(mg_ctx *ctx, uint8_t *fixed);
  * replaces the structure part of traitsMatrixBuilder::buildEpetraCrs
  * (traitsMatrixBuilder.hpp:366-441) + Epetra GlobalAssemble: every (row,col)
  * pair of dofs sharing a tetrahedron, columns sorted, structural zeros kept.
- * Also builds the element -> CSR slot map and the row-incidence records the
- * numeric assembly uses.  The pattern lives until mg_mesh_set,
+ * Also builds the row-incidence records the numeric assembly uses (the
+ * element -> CSR slot map is built on the way and released).  The pattern
+ * lives until mg_mesh_set,
  * mg_dofmap_build or (interior system) mg_dirichlet_set; calling this again
  * before that returns its sizes and keeps the assembled values.            */
 int mg_pattern_build(mg_ctx *ctx, int sys, int64_t *n_rows, int64_t *nnz);
 int mg_pattern_get(mg_ctx *ctx, int sys, int32_t *rowptr, int32_t *colidx);
 /* slot[e*nloc*nloc + i*nloc + j] = CSR index of (row dof j, column dof i) of
  * element e, in the column-major order of the reference's local matrix
- * (opColMajor, traitsMatrixBuilder.hpp:57-62); -1 for eliminated pairs.    */
+ * (opColMajor, traitsMatrixBuilder.hpp:57-62); -1 for eliminated pairs.
+ * Recomputed on the device for each call (the assembly does not use it, so
+ * it is not kept resident: 4 nloc^2 bytes per element).                    */
 int mg_slot_map_get(mg_ctx *ctx, int sys, int32_t *slot);
 
 /* ---- numeric assembly ------------------------------------------------------

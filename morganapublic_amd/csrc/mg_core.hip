@@ -20,7 +20,7 @@ void dfree(Ctx *c, void *p) {
 }
 
 void free_system(Ctx *c, System &S) {
-  for (void *p : {(void *)S.rowptr, (void *)S.colidx, (void *)S.diagpos, (void *)S.slot,
+  for (void *p : {(void *)S.rowptr, (void *)S.colidx, (void *)S.diagpos,
                   (void *)S.inc_ptr, S.inc_rec, (void *)S.vals64, (void *)S.vals32,
                   (void *)S.dinv64, (void *)S.dinv32, (void *)S.sdesc, (void *)S.cofs,
                   (void *)S.long_rows})
@@ -507,10 +507,12 @@ int mg_slot_map_get(mg_ctx *c, int sys, int32_t *slot) {
   API_BEGIN(c)
   MG_CHECK(sys == MG_SYS_FULL || sys == MG_SYS_INTERIOR, MG_ERR_ARG, "bad system");
   System &S = c->sys[sys];
-  MG_CHECK(S.built && S.slot, MG_ERR_STATE, "pattern not built");
+  MG_CHECK(S.built, MG_ERR_STATE, "pattern not built");
   MG_CHECK(slot, MG_ERR_ARG, "null output");
-  MG_HIP(hipMemcpyAsync(slot, S.slot, sizeof(int32_t) * c->nT * c->nloc * c->nloc,
-                        hipMemcpyDeviceToHost, c->stream));
+  const size_t ncon = (size_t)c->nT * c->nloc * c->nloc;
+  DevTmp<int32_t> ds(c, ncon);
+  slot_map(c, sys, ds);
+  MG_HIP(hipMemcpyAsync(slot, ds, sizeof(int32_t) * ncon, hipMemcpyDeviceToHost, c->stream));
   MG_HIP(hipStreamSynchronize(c->stream));
   API_END(c)
 }

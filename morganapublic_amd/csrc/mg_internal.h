@@ -97,7 +97,6 @@ struct System {
   int32_t *rowptr = nullptr;   // nrows + 1
   int32_t *colidx = nullptr;   // nnz
   int32_t *diagpos = nullptr;  // nrows, CSR index of the diagonal entry
-  int32_t *slot = nullptr;     // nT * nloc * nloc element -> CSR slot map
   int32_t *inc_ptr = nullptr;  // nrows + 1 (row -> incidences)
   void *inc_rec = nullptr;     // n_inc records (RecP1, RecP2 or RecP2c), sorted by (row, l, e)
   int rec_words = 0;           // int32 words per record: 2 (RecP1), 3 (RecP1w, RecP2c), 4 (RecP2), 6 (RecP2w)
@@ -259,6 +258,7 @@ void build_edges(Ctx *c);
 void build_dofs(Ctx *c);
 void build_dirichlet(Ctx *c, int n_geo, const int32_t *geo_ids);
 void build_pattern(Ctx *c, int sys);
+void slot_map(Ctx *c, int sys, int32_t *slot);  // element -> CSR position, computed on request
 void ensure_dinv(Ctx *c, int sys, int prec);
 void spmv_launch(Ctx *c, int sys, int prec, const void *x, void *y);
 

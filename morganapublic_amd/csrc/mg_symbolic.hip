@@ -492,19 +492,18 @@ static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvr
   add(nvrows, S.nrows, 4, erows, elpr, elpr);
 }
 
-void build_pattern(Ctx *c, int sys) {
-  System &S = c->sys[sys];
-  free_system(c, S);
+// The (row, column) pairs of every element contribution sorted and made
+// unique: slot[c] = CSR position of contribution c (-1: eliminated); with
+// cols_out, a new array of the column of every position (8 zeroed entries of
+// padding after it: the SpMV streams read a lane's 4 entries at once), and
+// with cnt the entries per row.  Returns nnz.
+static int64_t pattern_pass(Ctx *c, int sys, int32_t *slot, int32_t **cols_out, int32_t *cnt) {
   const int nl = c->nloc;
   const int32_t *map = sys == MG_SYS_FULL ? nullptr : c->sysmap;
   const int64_t nrows = sys == MG_SYS_FULL ? c->ndofs : c->nfree;
-  S.nrows = nrows;
-  MG_CHECK(nrows > 0, MG_ERR_STATE, "pattern: no rows");
   const int64_t ncon = c->nT * nl * nl;
-  MG_CHECK(ncon < (1ll << 31), MG_ERR_SIZE, "pattern: more than 2^31 element contributions");
   const int eb = bitlen((uint64_t)nrows * (uint64_t)nrows) + 1;
   const uint64_t invalid = (eb >= 64) ? ~0ull : ((1ull << eb) - 1);
-
   uint64_t *k0 = dalloc<uint64_t>(c, ncon), *k1 = dalloc<uint64_t>(c, ncon);
   int32_t *v0 = dalloc<int32_t>(c, ncon), *v1 = dalloc<int32_t>(c, ncon);
   k_pattern_keys<<<blocks_for(ncon, 256), 256, 0, c->stream>>>(c->nT, nl, c->dofs, map,
@@ -516,33 +515,60 @@ void build_pattern(Ctx *c, int sys) {
   int32_t *head = dalloc<int32_t>(c, ncon), *id = dalloc<int32_t>(c, ncon);
   k_heads_u64<<<blocks_for(ncon, 256), 256, 0, c->stream>>>(ncon, k1, invalid, head);
   inclusive_sum(c, head, id, ncon);
-  S.nnz = read_back(c, id + ncon - 1);
-  S.slot = dalloc<int32_t>(c, ncon);
-  // 8 entries of padding after each SpMV stream array: the streams read a
-  // lane's 4 entries at once (zeroed; never used as a column)
-  S.colidx = dalloc<int32_t>(c, S.nnz + 8);
-  MG_HIP(hipMemsetAsync(S.colidx + S.nnz, 0, 8 * sizeof(int32_t), c->stream));
-  int32_t *cnt = dalloc<int32_t>(c, nrows + 1);
-  S.rowptr = dalloc<int32_t>(c, nrows + 1);
-  MG_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (nrows + 1), c->stream));
+  const int64_t nnz = read_back(c, id + ncon - 1);
+  int32_t *cols = dalloc<int32_t>(c, nnz + 8);
+  MG_HIP(hipMemsetAsync(cols + nnz, 0, 8 * sizeof(int32_t), c->stream));
+  int32_t *tcnt = cnt ? nullptr : dalloc<int32_t>(c, nrows + 1);
+  if (!cnt) MG_HIP(hipMemsetAsync(tcnt, 0, sizeof(int32_t) * (nrows + 1), c->stream));
   k_pattern_scatter<<<blocks_for(ncon, 256), 256, 0, c->stream>>>(
-      ncon, k1, v1, id, head, (uint64_t)nrows, invalid, S.slot, S.colidx, cnt);
+      ncon, k1, v1, id, head, (uint64_t)nrows, invalid, slot, cols, cnt ? cnt : tcnt);
   MG_HIP(hipGetLastError());
+  MG_HIP(hipStreamSynchronize(c->stream));
+  for (void *p : {(void *)k1, (void *)v1, (void *)head, (void *)id, (void *)tcnt}) dfree(c, p);
+  if (cols_out)
+    *cols_out = cols;
+  else
+    dfree(c, cols);
+  return nnz;
+}
+
+// The element -> CSR position map of a built system (nT nloc^2 entries,
+// contribution c = (e nloc + i) nloc + j for field basis i, test basis j).
+// Recomputed on request: the assembly does not read it, so it is not kept
+// on the device between calls.
+void slot_map(Ctx *c, int sys, int32_t *slot) {
+  const int64_t nnz = pattern_pass(c, sys, slot, nullptr, nullptr);
+  MG_CHECK(nnz == c->sys[sys].nnz, MG_ERR_STATE, "slot map: pattern changed");
+}
+
+void build_pattern(Ctx *c, int sys) {
+  System &S = c->sys[sys];
+  free_system(c, S);
+  const int nl = c->nloc;
+  const int32_t *map = sys == MG_SYS_FULL ? nullptr : c->sysmap;
+  const int64_t nrows = sys == MG_SYS_FULL ? c->ndofs : c->nfree;
+  S.nrows = nrows;
+  MG_CHECK(nrows > 0, MG_ERR_STATE, "pattern: no rows");
+  const int64_t ncon = c->nT * nl * nl;
+  MG_CHECK(ncon < (1ll << 31), MG_ERR_SIZE, "pattern: more than 2^31 element contributions");
+  // the slot map lives only while the incidence records are built
+  int32_t *slot = dalloc<int32_t>(c, ncon);
+  int32_t *cnt = dalloc<int32_t>(c, nrows + 1);
+  MG_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (nrows + 1), c->stream));
+  S.nnz = pattern_pass(c, sys, slot, &S.colidx, cnt);
+  S.rowptr = dalloc<int32_t>(c, nrows + 1);
   exclusive_sum(c, cnt, S.rowptr, nrows + 1);
   S.diagpos = dalloc<int32_t>(c, nrows);
   k_diagpos<<<blocks_for(nrows, 256), 256, 0, c->stream>>>(nrows, S.rowptr, S.colidx, S.diagpos);
   MG_HIP(hipGetLastError());
-  for (void *p : {(void *)k1, (void *)v1, (void *)head, (void *)id}) dfree(c, p);
 
   // row incidence: (element, local basis) pairs grouped by row, ordered by
   // (local basis, element) inside a row
   const int64_t npair = c->nT * nl;
   const int eb2 = bitlen((uint64_t)nrows * nl) + 1;
   const uint64_t inv2 = (1ull << eb2) - 1;
-  k0 = dalloc<uint64_t>(c, npair);
-  k1 = dalloc<uint64_t>(c, npair);
-  v0 = dalloc<int32_t>(c, npair);
-  v1 = dalloc<int32_t>(c, npair);
+  uint64_t *k0 = dalloc<uint64_t>(c, npair), *k1 = dalloc<uint64_t>(c, npair);
+  int32_t *v0 = dalloc<int32_t>(c, npair), *v1 = dalloc<int32_t>(c, npair);
   k_inc_keys<<<blocks_for(npair, 256), 256, 0, c->stream>>>(c->nT, nl, c->dofs, map, inv2, k0, v0);
   MG_HIP(hipGetLastError());
   sort_pairs(c, k0, k1, v0, v1, npair, eb2);
@@ -578,36 +604,37 @@ void build_pattern(Ctx *c, int sys) {
     S.inc_rec = dalloc<RecP2c>(c, S.n_inc);
     S.rec_words = 3;
     S.rec_kind = kRecP2c;
-    k_inc_records_c<<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, S.slot, S.rowptr, S.diagpos,
+    k_inc_records_c<<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, slot, S.rowptr, S.diagpos,
                                                (RecP2c *)S.inc_rec, bad);
   } else if (nl == 10 && !wide) {
     S.inc_rec = dalloc<RecP2>(c, S.n_inc);
     S.rec_words = 4;
     S.rec_kind = kRecP2;
-    k_inc_records<10, RecP2><<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, S.slot, S.rowptr,
+    k_inc_records<10, RecP2><<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, slot, S.rowptr,
                                                         (RecP2 *)S.inc_rec, bad);
   } else if (nl == 10) {
     S.inc_rec = dalloc<RecP2w>(c, S.n_inc);
     S.rec_words = 6;
     S.rec_kind = kRecP2w;
-    k_inc_records<10, RecP2w><<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, S.slot, S.rowptr,
+    k_inc_records<10, RecP2w><<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, slot, S.rowptr,
                                                          (RecP2w *)S.inc_rec, bad);
   } else if (!wide) {
     S.inc_rec = dalloc<RecP1>(c, S.n_inc);
     S.rec_words = 2;
     S.rec_kind = kRecP1;
-    k_inc_records<4, RecP1><<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, S.slot, S.rowptr,
+    k_inc_records<4, RecP1><<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, slot, S.rowptr,
                                                        (RecP1 *)S.inc_rec, bad);
   } else {
     S.inc_rec = dalloc<RecP1w>(c, S.n_inc);
     S.rec_words = 3;
     S.rec_kind = kRecP1w;
-    k_inc_records<4, RecP1w><<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, S.slot, S.rowptr,
+    k_inc_records<4, RecP1w><<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, slot, S.rowptr,
                                                         (RecP1w *)S.inc_rec, bad);
   }
   MG_HIP(hipGetLastError());
   int hb = read_back(c, bad);
-  for (void *p : {(void *)k0, (void *)k1, (void *)v0, (void *)v1, (void *)cnt, (void *)bad})
+  for (void *p : {(void *)k0, (void *)k1, (void *)v0, (void *)v1, (void *)cnt, (void *)bad,
+                  (void *)slot})
     dfree(c, p);
   MG_CHECK(hb == 0, MG_ERR_STATE, "pattern: a record position does not fit its form");
   const int64_t nvrows =
