@@ -171,6 +171,19 @@ int mg_slot_map_get(mg_ctx *ctx, int sys, int32_t *slot);
  * prec = MG_PREC_F32 fills the FP32 copy (geometry in FP64, contraction and
  * accumulation in FP32).                                                    */
 int mg_assemble(mg_ctx *ctx, int sys, const mg_operator *op, int prec);
+/* Allocation-free variant of mg_assemble.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and the same values, bit for bit; the per-element factor records
+ * go to a static 128 MiB device array (one per GPU, held until the process
+ * ends) instead of context memory, and nothing is allocated or freed.
+ * MG_ERR_SIZE when the records need more than the array: P2 takes 48 bytes
+ * per element in FP64 (64 with a mass term) and half that in FP32, up to
+ * 2.79 M (2.09 M) elements; FP64 P1 takes none.  The pattern must have been
+ * built by mg_pattern_build (which allocates).  Every context on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams or
+ * host threads) race.                                                      */
+int mg_assemble_noalloc(mg_ctx *ctx, int sys, const mg_operator *op, int prec);
 int mg_values_get(mg_ctx *ctx, int sys, int prec, void *vals);
 /* device pointer of the nnz assembled values (double* or float*), in CSR
  * order, owned by the context until the pattern is rebuilt or destroyed     */

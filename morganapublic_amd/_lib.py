@@ -42,7 +42,7 @@ LOAD_CONST, LOAD_P0, LOAD_NODAL = 0, 1, 2
 EXPORTS = (
     "mg_ctx_create", "mg_ctx_destroy", "mg_last_error", "mg_ctx_stream", "mg_ctx_synchronize",
     "mg_copy_device",
-    "mg_assembly_seq",
+    "mg_assembly_seq", "mg_assemble_noalloc",
     "mg_mesh_set", "mg_field_set", "mg_field_set_device", "mg_dofmap_build", "mg_elem_dofs_get",
     "mg_edges_get", "mg_dirichlet_set", "mg_dirichlet_mask_get", "mg_pattern_build",
     "mg_pattern_get", "mg_slot_map_get", "mg_assemble", "mg_values_get", "mg_values_device",
@@ -124,6 +124,7 @@ def _declare(L):
         "mg_pattern_get": [_P, _I, _P, _P],
         "mg_slot_map_get": [_P, _I, _P],
         "mg_assemble": [_P, _I, C.POINTER(Operator), _I],
+        "mg_assemble_noalloc": [_P, _I, C.POINTER(Operator), _I],
         "mg_values_get": [_P, _I, _I, _P],
         "mg_values_device": [_P, _I, _I, C.POINTER(_P)],
         "mg_assembly_seq": [_P, _I, _I, C.POINTER(C.c_uint64)],

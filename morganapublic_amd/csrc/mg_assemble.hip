@@ -20,11 +20,37 @@
 // (deterministic, no global atomics), and the workgroup writes its contiguous
 // value range with coalesced nontemporal stores.
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "mg_internal.h"
 
+// Static scratch of the allocation-free assembly (mg_assemble_noalloc): the
+// per-element factor records live here instead of in context memory.  Every
+// GPU the library runs on holds its own copy for the life of the process.
+// Host code never uses this name as an address: the device address comes
+// from hipGetSymbolAddress (noalloc_init) and is what the kernels receive.
+static constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
 namespace mg {
+
+// Called by mg_ctx_create with the context's device current: looks up that
+// device's address of the scratch array once and keeps it per device.
+void noalloc_init(Ctx *c) {
+  static std::mutex mu;
+  static void *addr[64] = {};
+  std::lock_guard<std::mutex> lock(mu);
+  if (c->device < 64 && addr[c->device]) {
+    c->noalloc_scratch = addr[c->device];
+    return;
+  }
+  void *p = nullptr;
+  MG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)));
+  MG_CHECK(p != nullptr, MG_ERR_HIP, "noalloc: no device address for the scratch array");
+  if (c->device < 64) addr[c->device] = p;
+  c->noalloc_scratch = p;
+}
 
 __constant__ double c_tab64[7 * 100];
 __constant__ float c_tab32[7 * 100];
@@ -714,8 +740,11 @@ using IC = std::integral_constant<int, V>;
 template <bool V>
 using BC = std::integral_constant<bool, V>;
 
+// geo_buf: where the element factor records go (nullptr: the context's own
+// array, grown as needed)
 template <int NL, typename T, typename Rec>
-static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool elim) {
+static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool elim,
+                        T *geo_buf) {
   const bool stiff = op->terms & MG_TERM_STIFF, mass = op->terms & MG_TERM_MASS;
   const double ksc = stiff ? op->stiff_const : 0.0, kmc = mass ? op->mass_const : 0.0;
   const double *ksf = (stiff && op->stiff_coeff == MG_COEFF_P0) ? c->field[MG_FIELD_STIFF] : nullptr;
@@ -728,12 +757,12 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
   // element records
   const int stride = mass ? geo_stride<T, true>() : geo_stride<T, false>();
   const size_t gbytes = sizeof(T) * stride * (size_t)c->nT;
-  if (!inl && c->geom_bytes < gbytes) {
+  if (!inl && !geo_buf && c->geom_bytes < gbytes) {
     dfree(c, c->geom);
     c->geom = dalloc<char>(c, gbytes);
     c->geom_bytes = gbytes;
   }
-  T *geo = (T *)c->geom;
+  T *geo = geo_buf ? geo_buf : (T *)c->geom;
   if (c->nT > 0 && !inl) {
     if (mass)
       k_elem_factors<T, true><<<blocks_for(c->nT, 256), 256, 0, c->stream>>>(
@@ -804,20 +833,29 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
 static bool rule_ok(int r) { return r == 1 || r == 4 || r == 11 || r == 15; }
 
 template <typename T>
-static void assemble_t(Ctx *c, int sys, const mg_operator *op, int prec) {
+static void assemble_t(Ctx *c, int sys, const mg_operator *op, int prec, T *geo_buf) {
   System &S = c->sys[sys];
   const bool stiff = op->terms & MG_TERM_STIFF, mass = op->terms & MG_TERM_MASS;
   upload_tables(c, c->order, stiff ? op->stiff_rule : 0, mass ? op->mass_rule : 0, prec);
   T *vals = prec == MG_PREC_F64 ? (T *)S.vals64 : (T *)S.vals32;
   const bool elim = sys == MG_SYS_INTERIOR;
   switch (S.rec_kind) {
-    case kRecP2c: launch_rows<10, T, RecP2c>(c, S, op, vals, elim); break;
-    case kRecP2: launch_rows<10, T, RecP2>(c, S, op, vals, elim); break;
-    case kRecP2w: launch_rows<10, T, RecP2w>(c, S, op, vals, elim); break;
-    case kRecP1: launch_rows<4, T, RecP1>(c, S, op, vals, elim); break;
-    case kRecP1w: launch_rows<4, T, RecP1w>(c, S, op, vals, elim); break;
+    case kRecP2c: launch_rows<10, T, RecP2c>(c, S, op, vals, elim, geo_buf); break;
+    case kRecP2: launch_rows<10, T, RecP2>(c, S, op, vals, elim, geo_buf); break;
+    case kRecP2w: launch_rows<10, T, RecP2w>(c, S, op, vals, elim, geo_buf); break;
+    case kRecP1: launch_rows<4, T, RecP1>(c, S, op, vals, elim, geo_buf); break;
+    case kRecP1w: launch_rows<4, T, RecP1w>(c, S, op, vals, elim, geo_buf); break;
     default: MG_CHECK(false, MG_ERR_STATE, "assemble: no incidence records");
   }
+}
+
+// bytes of the element factor records an assembly writes (0: FP64 P1, which
+// forms them in place)
+static size_t factor_bytes(const Ctx *c, const mg_operator *op, int prec) {
+  const bool mass = op->terms & MG_TERM_MASS;
+  if (c->nloc == 4 && prec == MG_PREC_F64) return 0;
+  const size_t per = prec == MG_PREC_F64 ? 8 * (mass ? 8 : 6) : 4 * (mass ? 8 : 6);
+  return per * (size_t)c->nT;
 }
 
 // ---- adjoint element kernel ---------------------------------------------------
@@ -889,10 +927,10 @@ using namespace mg;
   }                                 \
   return MG_OK;
 
-extern "C" {
+namespace mg {
 
-int mg_assemble(mg_ctx *c, int sys, const mg_operator *op, int prec) {
-  API_BEGIN(c)
+// checks shared by mg_assemble and mg_assemble_noalloc; returns the operator key
+static OpKey assemble_checks(Ctx *c, int sys, const mg_operator *op, int prec) {
   MG_CHECK(op, MG_ERR_ARG, "assemble: null operator");
   MG_CHECK(sys == MG_SYS_FULL || sys == MG_SYS_INTERIOR, MG_ERR_ARG, "assemble: bad system");
   MG_CHECK(prec == MG_PREC_F64 || prec == MG_PREC_F32, MG_ERR_ARG, "assemble: bad precision");
@@ -909,6 +947,7 @@ int mg_assemble(mg_ctx *c, int sys, const mg_operator *op, int prec) {
     MG_CHECK(op->mass_coeff == MG_COEFF_CONST || c->field[MG_FIELD_MASS], MG_ERR_STATE,
              "assemble: mass P0 field not set");
   }
+  (void)S;
   OpKey key;
   key.terms = op->terms;
   if (op->terms & MG_TERM_STIFF) {
@@ -927,19 +966,48 @@ int mg_assemble(mg_ctx *c, int sys, const mg_operator *op, int prec) {
     else
       key.mass_const = op->mass_const;
   }
+  return key;
+}
+
+static void assemble_run(Ctx *c, int sys, const mg_operator *op, int prec, const OpKey &key,
+                         void *geo_buf) {
+  System &S = c->sys[sys];
   if (prec == MG_PREC_F64) {
-    assemble_t<double>(c, sys, op, prec);
+    assemble_t<double>(c, sys, op, prec, (double *)geo_buf);
     S.has64 = true;
     S.dinv64_ok = false;
     S.seq64 = ++c->asm_seq;
     S.key64 = key;
   } else {
-    assemble_t<float>(c, sys, op, prec);
+    assemble_t<float>(c, sys, op, prec, (float *)geo_buf);
     S.has32 = true;
     S.dinv32_ok = false;
     S.seq32 = ++c->asm_seq;
     S.key32 = key;
   }
+}
+
+}  // namespace mg
+
+extern "C" {
+
+int mg_assemble(mg_ctx *c, int sys, const mg_operator *op, int prec) {
+  API_BEGIN(c)
+  const OpKey key = assemble_checks(c, sys, op, prec);
+  assemble_run(c, sys, op, prec, key, nullptr);
+  API_END(c)
+}
+
+int mg_assemble_noalloc(mg_ctx *c, int sys, const mg_operator *op, int prec) {
+  API_BEGIN(c)
+  const OpKey key = assemble_checks(c, sys, op, prec);
+  const size_t need = factor_bytes(c, op, prec);
+  MG_CHECK(need <= kNoAllocScratchBytes, MG_ERR_SIZE,
+           "assemble_noalloc: the element factor records take " + std::to_string(need) +
+               " bytes, more than the " + std::to_string(kNoAllocScratchBytes) +
+               "-byte static scratch (use mg_assemble)");
+  MG_CHECK(c->noalloc_scratch, MG_ERR_STATE, "assemble_noalloc: no scratch address");
+  assemble_run(c, sys, op, prec, key, c->noalloc_scratch);
   API_END(c)
 }
 

@@ -301,6 +301,13 @@ int mg_ctx_create(int device, mg_ctx **out) {
     delete c;
     return MG_ERR_HIP;
   }
+  try {
+    noalloc_init(c);
+  } catch (const mg::Error &) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MG_ERR_HIP;
+  }
   *out = c;
   return MG_OK;
 }

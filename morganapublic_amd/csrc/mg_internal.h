@@ -181,6 +181,10 @@ struct Ctx {
   // scratch for library primitives (sorts, scans)
   void *tmp = nullptr;
   size_t tmp_bytes = 0;
+
+  // this device's address of the static scratch array of the
+  // allocation-free assembly (mg_assemble_noalloc), set by mg_ctx_create
+  void *noalloc_scratch = nullptr;
 };
 
 // ---- error helpers ----------------------------------------------------------
@@ -258,6 +262,7 @@ void build_edges(Ctx *c);
 void build_dofs(Ctx *c);
 void build_dirichlet(Ctx *c, int n_geo, const int32_t *geo_ids);
 void build_pattern(Ctx *c, int sys);
+void noalloc_init(Ctx *c);  // device address of the static scratch array (device current)
 void slot_map(Ctx *c, int sys, int32_t *slot);  // element -> CSR position, computed on request
 void ensure_dinv(Ctx *c, int sys, int prec);
 void spmv_launch(Ctx *c, int sys, int prec, const void *x, void *y);
