@@ -1,11 +1,12 @@
 // Numeric assembly of P1/P2 tetrahedral operators into a fixed CSR pattern,
 // and the adjoint element kernel for d(loss)/d(per-element coefficient).
 //
-// A P2 assembly is three launches, a P1 assembly two (FP32) or one (FP64).
+// A P2 assembly is three launches, a P1 assembly one.
 // k_elem_factors: one lane per element computes, in FP64 from the vertex
 // coordinates, the 6 edge factors S_ab = c vol grad(lambda_a).grad(lambda_b)
-// (and c_mass det with a mass term), stored per element (FP64 P1 rows form
-// them in place instead).  k_asm_rows, once over the vertex-dof rows and
+// (and c_mass det with a mass term), stored per element (P1 rows form them
+// in place instead, from the row block's vertex patch in LDS where it has
+// one).  k_asm_rows, once over the vertex-dof rows and
 // once over the P2 edge-dof rows: a workgroup owns consecutive CSR rows and
 // stages their value range in LDS; LPR lanes per row walk the row's
 // (element, local basis l) incidence records, whose element is relabelled by
@@ -95,6 +96,9 @@ template <>
 __device__ __forceinline__ float tab<float>(int i) {
   return c_tab32[i];
 }
+
+// (x, y) of a vertex as one 16-byte load at 8-byte alignment
+typedef double dbl2_a8 __attribute__((ext_vector_type(2), aligned(8)));
 
 struct GeomOut {
   double det;
@@ -195,9 +199,7 @@ __global__ __launch_bounds__(256) void k_elem_factors(
   const int64_t ec = min<int64_t>(e0 + threadIdx.x, nT - 1);
   const int4 t = reinterpret_cast<const int4 *>(tets)[ec];
   const int id[4] = {t.x, t.y, t.z, t.w};
-  // a vertex in two loads: (x, y) as one 16-byte load at 8-byte alignment,
-  // then z
-  typedef double dbl2_a8 __attribute__((ext_vector_type(2), aligned(8)));
+  // a vertex in two loads: (x, y), then z
   double P[4][3];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
@@ -235,6 +237,16 @@ __global__ __launch_bounds__(256) void k_elem_factors(
 }
 
 // What the P1 row kernels need to form element factors in place.
+// The vertex patches of a patched P1 segment (RecP1p records): per 64-row
+// block, its vertices (ptr, vid) and each row's own vertex (row_lv); slots is
+// the number of vertices whose coordinates the segment stages per block.
+struct PatchSrc {
+  const int32_t *ptr;
+  const int32_t *vid;
+  const uint16_t *row_lv;
+  int slots;
+};
+
 struct ElemSrc {
   const int32_t *tets;
   const double *xyz;
@@ -247,8 +259,11 @@ struct ElemSrc {
 // relabelled edge j is original edge sigma_l(j).  EdgePack.w[l] holds
 // sigma_l(j) in bits 3j..3j+2 (kernel arguments; selected with ALU ops, so no
 // memory access depends on l).
+// EdgePack.v4 holds, for P1 rows, the original vertex of relabelled vertex m
+// of row basis l in bits 2 (4 m + l) .. 2 (4 m + l) + 1.
 struct EdgePack {
   uint32_t w[10];
+  uint32_t v4;
 };
 
 static EdgePack pack_edges() {
@@ -257,6 +272,8 @@ static EdgePack pack_edges() {
   EdgePack p{};
   for (int l = 0; l < 10; ++l)
     for (int j = 0; j < 6; ++j) p.w[l] |= (uint32_t)(sigma[l][4 + j] - 4) << (3 * j);
+  for (int m = 0; m < 4; ++m)
+    for (int l = 0; l < 4; ++l) p.v4 |= (uint32_t)sigma[l][m] << (2 * (4 * m + l));
   return p;
 }
 
@@ -331,6 +348,30 @@ struct RecView<RecP1w, LROW> {
     const RecP1w q = rec[j];  // one dwordx3 load
     return RecView{q.el, {(uint32_t)q.pos[0] | ((uint32_t)q.pos[1] << 16),
                           (uint32_t)q.pos[2] | ((uint32_t)q.pos[3] << 16)}};
+  }
+};
+// patched P1: own column (basis 0) = the diagonal, `self`; lv(o) = patch index
+// of relabelled vertex o + 1
+template <int LROW>
+struct RecView<RecP1p, LROW> {
+  static constexpr int kElim = 0xFF;
+  int32_t el;
+  uint32_t lo, hi;
+  int self;
+  __device__ __forceinline__ int pos(int m) const {
+    return m == 0 ? self : (int)((lo >> (8 * (m - 1))) & 0xFFu);
+  }
+  __device__ __forceinline__ int lv(int o) const {
+    return (int)(o == 0 ? ((lo >> 24) | ((hi & 0xFu) << 8)) : ((hi >> (o == 1 ? 4 : 16)) & 0xFFFu));
+  }
+  __device__ __forceinline__ static RecView load(const RecP1p *__restrict__ rec, int j, int self) {
+    const RecP1p q = rec[j];  // one dwordx3 load
+    RecView r;
+    r.el = q.el;
+    r.lo = q.lo;
+    r.hi = q.hi;
+    r.self = self;
+    return r;
   }
 };
 // compact P2: the row's own column (basis LROW) is its diagonal, `self`
@@ -468,13 +509,17 @@ __device__ __forceinline__ void row_values(const T e[6], T md, T v[NL]) {
 // zeroes and accumulates its own row in the values array itself, in the same
 // incidence order, so the result is the staged form's bit for bit.
 template <int NL, int LROW, typename T, bool MASS, bool EXACT, bool ELIM, int ROWS, int LPR,
-          typename Rec, bool GACC = false>
-__global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
+          typename Rec, bool GACC = false, bool PATCH = false>
+// patched rows: at most 80 VGPRs, so 6 waves per SIMD fit, as many as the
+// LDS of 64-row blocks allows
+__global__ __launch_bounds__(ROWS *LPR, PATCH ? 6 : 1) void k_asm_rows(
     int32_t row_begin, int32_t row_end, int32_t dummy, EdgePack ep,
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ inc_ptr,
     const int32_t *__restrict__ diagpos, const Rec *__restrict__ rec, const T *__restrict__ geo,
-    ElemSrc es, T *__restrict__ vals) {
+    ElemSrc es, PatchSrc pc, T *__restrict__ vals) {
   static_assert(!GACC || LPR == 1, "global accumulation takes one lane per row");
+  static_assert(!PATCH || (NL == 4 && LROW == 0 && ROWS == kPatchRows && !GACC),
+                "vertex patches: P1 rows in kPatchRows-row staged blocks");
   extern __shared__ __align__(16) char lds_raw[];
   constexpr int NT = ROWS * LPR;
   constexpr int GS = geo_stride<T, MASS>();
@@ -490,17 +535,22 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
   // together after it (one round trip; otherwise the row offset is loaded
   // only after the loop guard has waited for the incidence range).
   // Measured at 64^3 P2: FP64 vertex rows are fastest pipelined.
-  // FP64 P1 rows form their element factors in place (no factor kernel, no
-  // factor array): measured 706 -> 593 us per 128^3 P1 assembly.  FP32 P1
-  // rows and all P2 rows read the factor records (in place: FP32 P1 +19%,
+  // P1 rows form their element factors in place (no factor kernel, no factor
+  // array: measured 706 -> 593 us per 128^3 P1 FP64 assembly with vertex
+  // gathers, before patches); P2 rows read the factor records (in place:
   // P2 FP64 the same, P2 FP32 +13%).
-  constexpr bool kInl = NL == 4 && sizeof(T) == 8;
+  // P1 rows (both precisions) form the factors in place, in the relabelled
+  // element's own vertex order (row vertex first), so the row's edge values
+  // come out of the arithmetic directly, without selecting them; patched rows
+  // read the vertex coordinates from the block's patch in LDS, so that a
+  // record and a coefficient are the only per-incidence global loads.
+  constexpr bool kInl = NL == 4;
   constexpr bool kChunked = kInl || LROW == 4 || sizeof(T) == 4;
   const int r = r0 + lane / LPR;
   const int rr = kChunked ? min(r, r1 - 1) : r;
   // compact records leave out the row's own column: its position is the
   // row's diagonal
-  constexpr bool kSelf = std::is_same<Rec, RecP2c>::value;
+  constexpr bool kSelf = std::is_same<Rec, RecP2c>::value || std::is_same<Rec, RecP1p>::value;
   using RV = RecView<Rec, LROW>;
   int base = 0, iend = 0, i = 0, self = 0;
   if (kChunked) {
@@ -508,6 +558,55 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
     iend = inc_ptr[rr + 1];
     i = inc_ptr[rr] + sub;
     if (kSelf) self = diagpos[rr] - s0 - base;  // position of the diagonal in the row
+  }
+  // vertex patch: coordinates as three arrays of pc.slots doubles after the
+  // staged entries; npm = the patch's last index (record indices are clamped
+  // to it: a row's records always point into its own block's patch)
+  double *PX = nullptr;
+  int lvr = 0, npm = 0;
+  // Patched rows: the lane's first KP incidences (records, then their
+  // coefficients) are requested before the patch is staged, so that their
+  // latency overlaps the staging and the barrier; a block then waits for
+  // about three memory round trips before it computes.
+  constexpr int KP = PATCH ? 6 : 1;
+  RV Rp[KP];
+  double ksp[KP], kmp[KP];
+  int lastp = 0;
+  if constexpr (PATCH) {
+    lastp = iend > i - sub ? iend - 1 : 0;
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      const int j = i + k * LPR;
+      Rp[k] = RV::load(rec, j < iend ? j : lastp, self);
+    }
+    PX = reinterpret_cast<double *>(lds_raw +
+                                    ((((size_t)dummy + 256) * sizeof(T) + 7) & ~(size_t)7));
+    const int pb = r0 / kPatchRows;
+    const int p0 = pc.ptr[pb], np = pc.ptr[pb + 1] - p0;
+    npm = max(np - 1, 0);
+    lvr = min((int)pc.row_lv[rr], npm);
+    // up to three vertices per lane with their loads in flight together
+    constexpr int KV = 3;
+    int vid[KV];
+#pragma unroll
+    for (int u = 0; u < KV; ++u) vid[u] = lane + u * NT < np ? pc.vid[p0 + lane + u * NT] : 0;
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      const int el = Rp[k].el >> 4;
+      ksp[k] = es.ksf ? es.ksf[el] : es.ksc;
+      kmp[k] = MASS ? (es.kmf ? es.kmf[el] : es.kmc) : 0.0;
+    }
+    auto put = [&](int p, int v) {
+      const dbl2_a8 q = *reinterpret_cast<const dbl2_a8 *>(es.xyz + 3 * (size_t)v);
+      const double z = es.xyz[3 * (size_t)v + 2];
+      PX[p] = q.x;
+      PX[pc.slots + p] = q.y;
+      PX[2 * pc.slots + p] = z;
+    };
+#pragma unroll
+    for (int u = 0; u < KV; ++u)
+      if (lane + u * NT < np) put(lane + u * NT, vid[u]);
+    for (int p = lane + KV * NT; p < np; p += NT) put(p, pc.vid[p0 + p]);
   }
   if constexpr (GACC) {
     // each lane clears its own row (it is the only one to touch it)
@@ -518,6 +617,12 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
   }
   __syncthreads();
   if (kChunked) asm volatile("" ::"v"(base), "v"(iend), "v"(i), "v"(self));
+  double own[3] = {0.0, 0.0, 0.0};  // the row's vertex (patched rows)
+  if constexpr (PATCH) {
+    own[0] = PX[lvr];
+    own[1] = PX[pc.slots + lvr];
+    own[2] = PX[2 * pc.slots + lvr];
+  }
   if (r < r1) {
     if (!kChunked) {
       base = rowptr[r] - s0;
@@ -640,14 +745,51 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
         if (LPR > 1) asm volatile("" ::: "memory");
       }
     };
-    if constexpr (kInl) {
+    if constexpr (PATCH) {
+      // the first KP incidences are in Rp/ksp/kmp already; each incidence's
+      // relabelled vertices 1-3 come from the patch, vertex 0 is the row's
+      bool first = true;
+      for (; i < iend; i += KP * LPR) {
+        if (!first) {
+#pragma unroll
+          for (int k = 0; k < KP; ++k) Rp[k] = RV::load(rec, at(i + k * LPR), self);
+#pragma unroll
+          for (int k = 0; k < KP; ++k) {
+            const int el = Rp[k].el >> 4;
+            ksp[k] = es.ksf ? es.ksf[el] : es.ksc;
+            kmp[k] = MASS ? (es.kmf ? es.kmf[el] : es.kmc) : 0.0;
+          }
+        }
+        first = false;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+          double P[4][3];
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) P[0][cc] = own[cc];
+#pragma unroll
+          for (int m = 1; m < 4; ++m) {
+            const int lv = min(Rp[k].lv(m - 1), npm);
+            P[m][0] = PX[lv];
+            P[m][1] = PX[pc.slots + lv];
+            P[m][2] = PX[2 * pc.slots + lv];
+          }
+          T f[8];
+          edge_factors<T, MASS>(P, ksp[k], kmp[k], f);
+          Slot A;
+          A.r = Rp[k];
+#pragma unroll
+          for (int j = 0; j < 6; ++j) A.e[j] = edge_needed<LROW, EXACT>(j) ? f[j] : (T)0;
+          A.md = MASS ? f[6] : (T)0;
+          add(A, i + k * LPR);
+        }
+      }
+    } else if constexpr (kInl) {
       // A lane takes 3 incidences at once (2: 626 us, 4: 764 us at 128^3):
       // their records, then vertex ids and coefficients, then the vertex
       // coordinates, and forms each element's factors with the arithmetic of
       // k_elem_factors (the values are bit-identical to the factor path).
       constexpr int KC = 3;
       const int4 *__restrict__ tets4 = reinterpret_cast<const int4 *>(es.tets);
-      typedef double dbl2_a8 __attribute__((ext_vector_type(2), aligned(8)));
       for (; i < iend; i += KC * LPR) {
         RV R[KC];
 #pragma unroll
@@ -661,30 +803,32 @@ __global__ __launch_bounds__(ROWS *LPR) void k_asm_rows(
           ks[k] = es.ksf ? es.ksf[el] : es.ksc;
           km[k] = MASS ? (es.kmf ? es.kmf[el] : es.kmc) : 0.0;
         }
+        // P[k][m]: relabelled vertex m of incidence k (m = 0: the row's)
         double P[KC][4][3];
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
+          const int l = R[k].el & 15;
           const int id[4] = {t[k].x, t[k].y, t[k].z, t[k].w};
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const dbl2_a8 q = *reinterpret_cast<const dbl2_a8 *>(es.xyz + 3 * (size_t)id[v]);
-            P[k][v][0] = q.x;
-            P[k][v][1] = q.y;
-            P[k][v][2] = es.xyz[3 * (size_t)id[v] + 2];
+          for (int m = 0; m < 4; ++m) {
+            const uint32_t o = (ep.v4 >> (2 * (4 * m + l))) & 3u;
+            const int a = (o & 1u) ? id[1] : id[0], b = (o & 1u) ? id[3] : id[2];
+            const int v = (o & 2u) ? b : a;
+            const dbl2_a8 q = *reinterpret_cast<const dbl2_a8 *>(es.xyz + 3 * (size_t)v);
+            P[k][m][0] = q.x;
+            P[k][m][1] = q.y;
+            P[k][m][2] = es.xyz[3 * (size_t)v + 2];
           }
         }
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
+          // the factors of the relabelled element are its row's edge values
           T f[8];
           edge_factors<T, MASS>(P[k], ks[k], km[k], f);
-          const uint32_t w = edge_word(R[k].el & 15);
           Slot A;
           A.r = R[k];
 #pragma unroll
-          for (int j = 0; j < 6; ++j)
-            A.e[j] = edge_needed<LROW, EXACT>(j)
-                         ? pick6v(f[0], f[1], f[2], f[3], f[4], f[5], (w >> (3 * j)) & 7u)
-                         : (T)0;
+          for (int j = 0; j < 6; ++j) A.e[j] = edge_needed<LROW, EXACT>(j) ? f[j] : (T)0;
           A.md = MASS ? f[6] : (T)0;
           add(A, i + k * LPR);
         }
@@ -740,6 +884,10 @@ using IC = std::integral_constant<int, V>;
 template <bool V>
 using BC = std::integral_constant<bool, V>;
 
+// Whether an assembly of S in this precision needs the factor kernel: P1
+// rows form the factors in place.
+static bool needs_factors(const System &, int nl, size_t) { return nl != 4; }
+
 // geo_buf: where the element factor records go (nullptr: the context's own
 // array, grown as needed)
 template <int NL, typename T, typename Rec>
@@ -753,7 +901,9 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
                      (!mass || rule_exact(c->order, MG_TERM_MASS, op->mass_rule));
   static const EdgePack ep = pack_edges();
   const ElemSrc es{c->tets, c->xyz, ksf, kmf, ksc, kmc};
-  const bool inl = NL == 4 && sizeof(T) == 8;  // as kInl in k_asm_rows
+  // no factor kernel when every segment forms the factors in place (as kInl
+  // in k_asm_rows)
+  const bool inl = !needs_factors(S, NL, sizeof(T));
   // element records
   const int stride = mass ? geo_stride<T, true>() : geo_stride<T, false>();
   const size_t gbytes = sizeof(T) * stride * (size_t)c->nT;
@@ -775,45 +925,66 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
   const Rec *rec = (const Rec *)S.inc_rec;
   for (const RowSegment &sg : S.segs) {
     const int nblk = (sg.row_end - sg.row_begin + sg.rows - 1) / sg.rows;
-    const size_t lds = sg.gacc ? 0 : sizeof(T) * (size_t)(sg.lds_slots + 256);
-    auto go = [&](auto lr, auto ms, auto ex, auto rw, auto lp, auto ga) {
+    // staged entries, the 256-slot region of eliminated columns, then (at
+    // 8-byte alignment) the vertex patch coordinates
+    size_t lds = sg.gacc ? 0 : sizeof(T) * (size_t)(sg.lds_slots + 256);
+    if (sg.patch) lds = ((lds + 7) & ~(size_t)7) + 24 * (size_t)sg.patch;
+    const PatchSrc pc{S.patch_ptr, S.patch_vid, S.row_lv, sg.patch};
+    auto go = [&](auto lr, auto ms, auto ex, auto rw, auto lp, auto ga, auto pt) {
       if (elim)
         k_asm_rows<NL, decltype(lr)::value, T, decltype(ms)::value, decltype(ex)::value, true,
-                   decltype(rw)::value, decltype(lp)::value, Rec, decltype(ga)::value>
+                   decltype(rw)::value, decltype(lp)::value, Rec, decltype(ga)::value,
+                   decltype(pt)::value>
             <<<nblk, decltype(rw)::value * decltype(lp)::value, lds, c->stream>>>(
                 sg.row_begin, sg.row_end, sg.lds_slots, ep, S.rowptr, S.inc_ptr, S.diagpos, rec, geo,
-                es, vals);
+                es, pc, vals);
       else
         k_asm_rows<NL, decltype(lr)::value, T, decltype(ms)::value, decltype(ex)::value, false,
-                   decltype(rw)::value, decltype(lp)::value, Rec, decltype(ga)::value>
-          <<<nblk, decltype(rw)::value * decltype(lp)::value, lds, c->stream>>>(
-              sg.row_begin, sg.row_end, sg.lds_slots, ep, S.rowptr, S.inc_ptr, S.diagpos, rec, geo,
-                es, vals);
+                   decltype(rw)::value, decltype(lp)::value, Rec, decltype(ga)::value,
+                   decltype(pt)::value>
+            <<<nblk, decltype(rw)::value * decltype(lp)::value, lds, c->stream>>>(
+                sg.row_begin, sg.row_end, sg.lds_slots, ep, S.rowptr, S.inc_ptr, S.diagpos, rec, geo,
+                es, pc, vals);
     };
     // vertex-dof rows: 64 rows per workgroup, 1/2/4 lanes per row; P2 edge-dof
     // rows: 64/128/256 rows per workgroup, 1/2 lanes per row; blocks past the
     // LDS budget: 64 rows, one lane per row, accumulated in place
     auto shape = [&](auto lr, auto ms, auto ex) {
+      const BC<false> st{};
       if (sg.gacc) {
-        go(lr, ms, ex, IC<64>{}, IC<1>{}, BC<true>{});
+        go(lr, ms, ex, IC<64>{}, IC<1>{}, BC<true>{}, st);
         return;
       }
-      const BC<false> st{};
       if constexpr (decltype(lr)::value == 0) {
         const int lpr = sizeof(T) == 4 ? sg.lpr32 : sg.lpr;
+        if constexpr (std::is_same<Rec, RecP1p>::value) {
+          if (sg.patch) {
+            const BC<true> pt{};
+            if (lpr == 4)
+              go(lr, ms, ex, IC<kPatchRows>{}, IC<4>{}, st, pt);
+            else if (lpr == 2)
+              go(lr, ms, ex, IC<kPatchRows>{}, IC<2>{}, st, pt);
+            else
+              go(lr, ms, ex, IC<kPatchRows>{}, IC<1>{}, st, pt);
+            return;
+          }
+        }
         if (lpr == 4)
-          go(lr, ms, ex, IC<64>{}, IC<4>{}, st);
+          go(lr, ms, ex, IC<64>{}, IC<4>{}, st, st);
         else if (lpr == 2)
-          go(lr, ms, ex, IC<64>{}, IC<2>{}, st);
+          go(lr, ms, ex, IC<64>{}, IC<2>{}, st, st);
         else
-          go(lr, ms, ex, IC<64>{}, IC<1>{}, st);
+          go(lr, ms, ex, IC<64>{}, IC<1>{}, st, st);
       } else {
         if (sg.rows == 256)
-          sg.lpr == 2 ? go(lr, ms, ex, IC<256>{}, IC<2>{}, st) : go(lr, ms, ex, IC<256>{}, IC<1>{}, st);
+          sg.lpr == 2 ? go(lr, ms, ex, IC<256>{}, IC<2>{}, st, st)
+                      : go(lr, ms, ex, IC<256>{}, IC<1>{}, st, st);
         else if (sg.rows == 128)
-          sg.lpr == 2 ? go(lr, ms, ex, IC<128>{}, IC<2>{}, st) : go(lr, ms, ex, IC<128>{}, IC<1>{}, st);
+          sg.lpr == 2 ? go(lr, ms, ex, IC<128>{}, IC<2>{}, st, st)
+                      : go(lr, ms, ex, IC<128>{}, IC<1>{}, st, st);
         else
-          sg.lpr == 2 ? go(lr, ms, ex, IC<64>{}, IC<2>{}, st) : go(lr, ms, ex, IC<64>{}, IC<1>{}, st);
+          sg.lpr == 2 ? go(lr, ms, ex, IC<64>{}, IC<2>{}, st, st)
+                      : go(lr, ms, ex, IC<64>{}, IC<1>{}, st, st);
       }
     };
     auto terms = [&](auto lr) {
@@ -845,15 +1016,16 @@ static void assemble_t(Ctx *c, int sys, const mg_operator *op, int prec, T *geo_
     case kRecP2w: launch_rows<10, T, RecP2w>(c, S, op, vals, elim, geo_buf); break;
     case kRecP1: launch_rows<4, T, RecP1>(c, S, op, vals, elim, geo_buf); break;
     case kRecP1w: launch_rows<4, T, RecP1w>(c, S, op, vals, elim, geo_buf); break;
+    case kRecP1p: launch_rows<4, T, RecP1p>(c, S, op, vals, elim, geo_buf); break;
     default: MG_CHECK(false, MG_ERR_STATE, "assemble: no incidence records");
   }
 }
 
-// bytes of the element factor records an assembly writes (0: FP64 P1, which
-// forms them in place)
-static size_t factor_bytes(const Ctx *c, const mg_operator *op, int prec) {
+// bytes of the element factor records an assembly writes (0: the rows form
+// them in place)
+static size_t factor_bytes(const Ctx *c, int sys, const mg_operator *op, int prec) {
   const bool mass = op->terms & MG_TERM_MASS;
-  if (c->nloc == 4 && prec == MG_PREC_F64) return 0;
+  if (!needs_factors(c->sys[sys], c->nloc, prec == MG_PREC_F64 ? 8 : 4)) return 0;
   const size_t per = prec == MG_PREC_F64 ? 8 * (mass ? 8 : 6) : 4 * (mass ? 8 : 6);
   return per * (size_t)c->nT;
 }
@@ -1001,7 +1173,7 @@ int mg_assemble(mg_ctx *c, int sys, const mg_operator *op, int prec) {
 int mg_assemble_noalloc(mg_ctx *c, int sys, const mg_operator *op, int prec) {
   API_BEGIN(c)
   const OpKey key = assemble_checks(c, sys, op, prec);
-  const size_t need = factor_bytes(c, op, prec);
+  const size_t need = factor_bytes(c, sys, op, prec);
   MG_CHECK(need <= kNoAllocScratchBytes, MG_ERR_SIZE,
            "assemble_noalloc: the element factor records take " + std::to_string(need) +
                " bytes, more than the " + std::to_string(kNoAllocScratchBytes) +

@@ -45,7 +45,25 @@ struct alignas(4) RecP1w {
   int32_t el;
   uint16_t pos[4];
 };
-enum RecKind { kRecP1 = 1, kRecP1w, kRecP2, kRecP2c, kRecP2w };
+// Patched P1 record (12 bytes, rows of <= 254 entries): the row's own column
+// (relabelled basis 0) is its diagonal; bits 0-23 of (lo, hi) hold the 8-bit
+// positions of relabelled bases 1-3 (0xFF: eliminated column), bits 24-59
+// the 12-bit indices, in the row block's vertex patch, of relabelled
+// vertices 1-3 (all zero for a row block without a patch).
+struct alignas(4) RecP1p {
+  int32_t el;
+  uint32_t lo, hi;
+};
+enum RecKind { kRecP1 = 1, kRecP1w, kRecP2, kRecP2c, kRecP2w, kRecP1p };
+
+// Vertex patches of P1 row blocks: the sorted distinct vertices of the
+// elements incident to kPatchRows consecutive rows, whose coordinates a row
+// block stages in LDS.  A block gets a patch when its vertex references fit
+// the LDS sort (kPatchKeys), it has at most kPatchMax distinct vertices and
+// its entries and coordinates fit the assembly's LDS budget.
+constexpr int kPatchRows = 64;
+constexpr int kPatchMax = 2048;
+constexpr int kPatchKeys = 8192;
 
 // Rows of one assembly launch: blocks of `rows` rows, each staging its CSR
 // entries in LDS; lrow is the local basis every incidence of these rows is relabelled to
@@ -57,6 +75,9 @@ struct RowSegment {
   // 1: the block's entries exceed the LDS budget and are accumulated in the
   // values array itself, one lane per row (no staging)
   int32_t gacc;
+  // > 0: every block has a vertex patch of at most this many vertices, staged
+  // in LDS after the entries (P1 rows)
+  int32_t patch;
 };
 
 constexpr int kSpmvRows = 64;  // rows per SpMV block (128 for short rows)
@@ -99,9 +120,15 @@ struct System {
   int32_t *diagpos = nullptr;  // nrows, CSR index of the diagonal entry
   int32_t *inc_ptr = nullptr;  // nrows + 1 (row -> incidences)
   void *inc_rec = nullptr;     // n_inc records (RecP1, RecP2 or RecP2c), sorted by (row, l, e)
-  int rec_words = 0;           // int32 words per record: 2 (RecP1), 3 (RecP1w, RecP2c), 4 (RecP2), 6 (RecP2w)
+  int rec_words = 0;           // int32 words per record: 2 (RecP1), 3 (RecP1w, RecP1p, RecP2c), 4 (RecP2), 6 (RecP2w)
   int rec_kind = 0;            // RecKind
   std::vector<RowSegment> segs;
+  // vertex patches (RecP1p systems): block b = rows [64 b, 64 b + 64) has
+  // vertices patch_vid[patch_ptr[b] .. patch_ptr[b + 1]) (none: equal
+  // pointers); row_lv[r] is the index of row r's own vertex in its patch
+  int32_t *patch_ptr = nullptr;
+  int32_t *patch_vid = nullptr;
+  uint16_t *row_lv = nullptr;
   // SpMV row blocks: each holds <= kSpmvNnz entries and <= kSpmvRows rows
   BlkDesc *sdesc = nullptr;
   int32_t n_sblk = 0;

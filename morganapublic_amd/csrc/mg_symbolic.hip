@@ -8,6 +8,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <type_traits>
 
@@ -436,18 +437,204 @@ static int env_int(const char *name, int dflt) {
   return v ? atoi(v) : dflt;
 }
 
+// Patched P1 records: the 8-bit positions of relabelled bases 1-3; the own
+// column (basis 0) must be the row's diagonal.  k_patch adds the patch indices.
+__global__ void k_inc_records_p1p(int64_t n_inc, const uint64_t *__restrict__ keys,
+                                  const int32_t *__restrict__ vals,
+                                  const int32_t *__restrict__ slot,
+                                  const int32_t *__restrict__ rowptr,
+                                  const int32_t *__restrict__ diagpos, RecP1p *__restrict__ rec,
+                                  int *__restrict__ bad) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_inc) return;
+  const int32_t v = vals[i];
+  const int64_t e = v >> 4;
+  const int l = v & 15;
+  const int64_t row = (int64_t)(keys[i] / 4);
+  const int32_t base = rowptr[row];
+  uint32_t w = 0;
+  for (int m = 0; m < 4; ++m) {
+    const int32_t s = slot[e * 16 + c_sigma[l][m] * 4 + l];
+    if (m == 0) {
+      if (s != diagpos[row]) atomicAdd(bad, 1);
+      continue;
+    }
+    int p = s < 0 ? 0xFF : s - base;
+    if (s >= 0 && (p < 0 || p >= 0xFF)) {
+      atomicAdd(bad, 1);
+      p = 0xFF;
+    }
+    w |= (uint32_t)p << (8 * (m - 1));
+  }
+  rec[i] = RecP1p{v, w, 0u};
+}
+
+// Vertex patch of row block b (rows kPatchRows b ..): the vertices of the
+// elements of the block's incidences, sorted in LDS (bitonic) and made
+// distinct.  FILL false: cnt[b] = the number of distinct vertices (-1: more
+// than kPatchKeys references).  FILL true, for blocks given a patch
+// (pptr[b + 1] > pptr[b]): the vertices go to vid[pptr[b] ..], every record of
+// the block gets the patch indices of its relabelled element's vertices 1-3,
+// and row_lv[r] the index of row r's own vertex (relabelled vertex 0).
+__device__ __forceinline__ int patch_find(const int32_t *u, int n, int32_t v) {
+  int lo = 0, hi = n;  // first index with u[index] >= v
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (u[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_patch(int64_t nrows, const int32_t *__restrict__ inc_ptr,
+                                               RecP1p *__restrict__ rec,
+                                               const int32_t *__restrict__ tets,
+                                               const int32_t *__restrict__ pptr,
+                                               int32_t *__restrict__ out,
+                                               uint16_t *__restrict__ row_lv) {
+  __shared__ int32_t key[kPatchKeys];
+  __shared__ int32_t uniq[kPatchMax];
+  __shared__ int32_t part[257];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int64_t r0 = (int64_t)b * kPatchRows, r1 = min<int64_t>(r0 + kPatchRows, nrows);
+  const int i0 = inc_ptr[r0], i1 = inc_ptr[r1];
+  const int n4 = 4 * (i1 - i0);
+  if (FILL) {
+    if (pptr[b + 1] == pptr[b]) return;
+  } else if (n4 > kPatchKeys) {
+    if (t == 0) out[b] = -1;
+    return;
+  }
+  int P = 1;
+  while (P < n4) P <<= 1;
+  for (int j = t; j < P; j += 256)
+    key[j] = j < n4 ? tets[4 * (size_t)(rec[i0 + (j >> 2)].el >> 4) + (j & 3)] : INT_MAX;
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = t; i < P; i += 256) {
+        const int o = i ^ j;
+        if (o > i) {
+          const int32_t a = key[i], c = key[o];
+          if ((a > c) == ((i & k) == 0)) {
+            key[i] = c;
+            key[o] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  // distinct values: thread t counts the first occurrences in its run of
+  // consecutive keys, an exclusive scan of the counts gives their indices
+  const int per = (P + 255) / 256;
+  const int a0 = min(t * per, n4), a1 = min(a0 + per, n4);
+  int nu = 0;
+  for (int i = a0; i < a1; ++i) nu += (i == 0 || key[i] != key[i - 1]);
+  part[t + 1] = nu;
+  __syncthreads();
+  if (t == 0) {
+    part[0] = 0;
+    for (int k = 1; k <= 256; ++k) part[k] += part[k - 1];
+  }
+  __syncthreads();
+  const int U = part[256];
+  if (!FILL) {
+    if (t == 0) out[b] = U;
+    return;
+  }
+  int u = part[t];
+  for (int i = a0; i < a1; ++i)
+    if (i == 0 || key[i] != key[i - 1]) {
+      uniq[u] = key[i];
+      out[pptr[b] + u] = key[i];
+      ++u;
+    }
+  __syncthreads();
+  for (int j = i0 + t; j < i1; j += 256) {
+    RecP1p q = rec[j];
+    const int e = q.el >> 4, l = q.el & 15;
+    uint32_t lv[3];
+    for (int m = 1; m < 4; ++m)
+      lv[m - 1] = (uint32_t)patch_find(uniq, U, tets[4 * (size_t)e + c_sigma[l][m]]);
+    q.lo = (q.lo & 0xFFFFFFu) | (lv[0] << 24);
+    q.hi = (lv[0] >> 8) | (lv[1] << 4) | (lv[2] << 16);
+    rec[j] = q;
+  }
+  for (int64_t r = r0 + t; r < r1; r += 256) {
+    const int j = inc_ptr[r];
+    int own = 0;
+    if (j < inc_ptr[r + 1]) {
+      const int32_t el = rec[j].el;
+      own = patch_find(uniq, U, tets[4 * (size_t)(el >> 4) + (el & 15)]);
+    }
+    row_lv[r] = (uint16_t)own;
+  }
+}
+
+// Vertex patches of a RecP1p system; returns the patch size of every row
+// block (0: no patch).  A block gets one when its distinct vertices fit the
+// index form and its staged entries and coordinates each take at most half
+// of the 64 KiB the assembly stages per block (MGFE_PATCH=2: patched
+// records, no patches; MGFE_PATCH_MAX: a lower vertex limit, which mixes
+// blocks with and without patches on small meshes).
+static std::vector<int32_t> build_patches(Ctx *c, System &S, const std::vector<int32_t> &rp) {
+  const int64_t nb = (S.nrows + kPatchRows - 1) / kPatchRows;
+  std::vector<int32_t> hc(nb, 0), pp(nb + 1, 0);
+  const bool on = env_int("MGFE_PATCH", 1) == 1;
+  const int umax = std::min(kPatchMax, std::max(1, env_int("MGFE_PATCH_MAX", kPatchMax)));
+  if (on) {
+    int32_t *cnt = dalloc<int32_t>(c, nb);
+    k_patch<false><<<(unsigned)nb, 256, 0, c->stream>>>(S.nrows, S.inc_ptr, (RecP1p *)S.inc_rec,
+                                                        c->tets, nullptr, cnt, nullptr);
+    MG_HIP(hipGetLastError());
+    MG_HIP(hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * nb, hipMemcpyDeviceToHost,
+                          c->stream));
+    MG_HIP(hipStreamSynchronize(c->stream));
+    dfree(c, cnt);
+  }
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t r0 = b * kPatchRows, r1 = std::min<int64_t>(r0 + kPatchRows, S.nrows);
+    const int ent = rp[r1] - rp[r0];
+    const int u = hc[b];
+    const bool ok = u > 0 && u <= umax && (ent + 256) * 8 <= 32 * 1024 && u * 24 <= 32 * 1024;
+    hc[b] = ok ? u : 0;
+    pp[b + 1] = pp[b] + hc[b];
+  }
+  S.patch_ptr = dalloc<int32_t>(c, nb + 1);
+  S.patch_vid = dalloc<int32_t>(c, std::max<int64_t>(pp[nb], 1));
+  S.row_lv = dalloc<uint16_t>(c, S.nrows);
+  MG_HIP(hipMemcpyAsync(S.patch_ptr, pp.data(), sizeof(int32_t) * (nb + 1), hipMemcpyHostToDevice,
+                        c->stream));
+  MG_HIP(hipMemsetAsync(S.row_lv, 0, sizeof(uint16_t) * S.nrows, c->stream));
+  if (pp[nb] > 0) {
+    k_patch<true><<<(unsigned)nb, 256, 0, c->stream>>>(S.nrows, S.inc_ptr, (RecP1p *)S.inc_rec,
+                                                       c->tets, S.patch_ptr, S.patch_vid,
+                                                       S.row_lv);
+    MG_HIP(hipGetLastError());
+  }
+  MG_HIP(hipStreamSynchronize(c->stream));
+  return hc;
+}
+
+
 // LDS budget of a staged row block: its entries plus the 256-slot region
 // that takes eliminated columns, in FP64, within 64 KiB
 static bool block_fits(int entries) { return (entries + 256) * 8 <= 64 * 1024; }
 
-static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvrows, int nl) {
+// psz: the vertex patch size of every kPatchRows-row block (P1 records with
+// patches), nullptr otherwise
+static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvrows, int nl,
+                          const std::vector<int32_t> *psz) {
   S.segs.clear();
   auto push = [&](int64_t a, int64_t b, int lrow, int rows, int lpr, int lpr32, bool gacc) {
     if (b <= a) return;
     int mx = 0;
     for (int64_t r = a; r < b; r += rows) mx = std::max(mx, rp[std::min<int64_t>(r + rows, b)] - rp[r]);
     S.segs.push_back(RowSegment{(int32_t)a, (int32_t)b, gacc ? 0 : mx, lrow, rows,
-                                gacc ? 1 : lpr, gacc ? 1 : lpr32, gacc ? 1 : 0});
+                                gacc ? 1 : lpr, gacc ? 1 : lpr32, gacc ? 1 : 0, 0});
   };
   auto add = [&](int64_t a, int64_t b, int lrow, int rows, int lpr, int lpr32) {
     if (b <= a) return;
@@ -486,7 +673,37 @@ static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvr
   const int vl = env_int("MGFE_VERTEX_LPR", 4),
             vl32 = env_int("MGFE_VERTEX_LPR32", nl == 10 ? 2 : 4),
             er = env_int("MGFE_EDGE_ROWS", 128), el = env_int("MGFE_EDGE_LPR", 2);
-  add(0, nvrows, 0, 64, pick(vl, 1, 2, 4) ? vl : 4, pick(vl32, 1, 2, 4) ? vl32 : (nl == 10 ? 2 : 4));
+  const int vlpr = pick(vl, 1, 2, 4) ? vl : 4;
+  const int vlpr32 = pick(vl32, 1, 2, 4) ? vl32 : (nl == 10 ? 2 : 4);
+  if (psz) {
+    // P1 rows in kPatchRows-row blocks, segments split where a block's kind
+    // changes: staged (0), accumulated in place (1), staged with a patch (2)
+    auto kind = [&](int64_t r) {
+      if (!block_fits(rp[std::min<int64_t>(r + kPatchRows, nvrows)] - rp[r])) return 1;
+      return (*psz)[r / kPatchRows] > 0 ? 2 : 0;
+    };
+    auto flush = [&](int64_t a, int64_t b, int k) {
+      push(a, b, 0, kPatchRows, vlpr, vlpr32, k == 1);
+      if (k == 2) {
+        int mp = 0;
+        for (int64_t r = a; r < b; r += kPatchRows) mp = std::max(mp, (*psz)[r / kPatchRows]);
+        S.segs.back().patch = mp;
+      }
+    };
+    int64_t run = 0;
+    int rk = nvrows > 0 ? kind(0) : 0;
+    for (int64_t r = 0; r < nvrows; r += kPatchRows) {
+      const int k = kind(r);
+      if (k != rk) {
+        flush(run, r, rk);
+        run = r;
+        rk = k;
+      }
+    }
+    flush(run, nvrows, rk);
+    return;
+  }
+  add(0, nvrows, 0, 64, vlpr, vlpr32);
   const int erows = pick(er, 64, 128, 256) ? er : 128;
   const int elpr = (el == 1 || el == 2) ? el : 2;
   add(nvrows, S.nrows, 4, erows, elpr, elpr);
@@ -598,6 +815,9 @@ void build_pattern(Ctx *c, int sys) {
   // positions up to 254, uint16 positions beyond (up to 65534)
   MG_CHECK(maxnnz <= 65534, MG_ERR_SIZE, "pattern: a row has more than 65534 entries");
   const bool compact = nl == 10 && maxnnz <= 127 && env_int("MGFE_REC16", 0) == 0;
+  // P1 rows up to 254 entries: patched records (MGFE_PATCH=0: the 8-byte
+  // form without patches)
+  const bool patched = nl == 4 && maxnnz <= 254 && env_int("MGFE_PATCH", 1) != 0;
   const bool wide = maxnnz > 254;
   const int gi = blocks_for(S.n_inc, 256);
   if (compact) {
@@ -618,6 +838,12 @@ void build_pattern(Ctx *c, int sys) {
     S.rec_kind = kRecP2w;
     k_inc_records<10, RecP2w><<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, slot, S.rowptr,
                                                          (RecP2w *)S.inc_rec, bad);
+  } else if (patched) {
+    S.inc_rec = dalloc<RecP1p>(c, S.n_inc);
+    S.rec_words = 3;
+    S.rec_kind = kRecP1p;
+    k_inc_records_p1p<<<gi, 256, 0, c->stream>>>(S.n_inc, k1, v1, slot, S.rowptr, S.diagpos,
+                                                 (RecP1p *)S.inc_rec, bad);
   } else if (!wide) {
     S.inc_rec = dalloc<RecP1>(c, S.n_inc);
     S.rec_words = 2;
@@ -639,7 +865,12 @@ void build_pattern(Ctx *c, int sys) {
   MG_CHECK(hb == 0, MG_ERR_STATE, "pattern: a record position does not fit its form");
   const int64_t nvrows =
       nl == 4 ? nrows : (sys == MG_SYS_FULL ? c->nV : c->nfree_vertex);
-  make_segments(S, rp, nvrows, nl);
+  if (S.rec_kind == kRecP1p) {
+    const std::vector<int32_t> psz = build_patches(c, S, rp);
+    make_segments(S, rp, nvrows, nl, &psz);
+  } else {
+    make_segments(S, rp, nvrows, nl, nullptr);
+  }
   {
     // SpMV row blocks
     std::vector<int32_t> sb;

@@ -515,3 +515,33 @@ def test_pcg_device_unaligned_vectors(ctx, mode):
     for x in out:
         assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) <= 1.5e-10
     assert np.linalg.norm(out[0] - out[1]) <= 1e-8 * np.linalg.norm(out[0])
+
+
+# P1 record forms: (MGFE_PATCH, MGFE_PATCH_MAX): 8-byte records; patched
+# records without patches; patches everywhere; patches on some row blocks
+PATCHES = [("0", None), ("2", None), ("1", None), ("1", "90"), ("1", "160"), ("1", "260")]
+
+
+@pytest.mark.parametrize("terms,rs,rm", [("stiff", 1, 4), ("both", 1, 4), ("both", 4, 1)])
+@pytest.mark.parametrize("prec", [L.PREC_F64, L.PREC_F32])
+@pytest.mark.parametrize("sysid", [L.SYS_FULL, L.SYS_INTERIOR])
+def test_p1_vertex_patches_bit_identical(ctx, monkeypatch, terms, rs, rm, prec, sysid):
+    # a row block's element vertices staged in LDS (vertex patch) instead of
+    # gathered per incidence: the same factors from the same coordinates in
+    # the same order, for blocks with and without a patch in one launch
+    m = W.kuhn_cube(13, jitter=0.2, seed=21)
+    k = np.exp(W.stream_normal(0, W.STREAM_K, m.nT))
+    t = L.TERM_STIFF | (L.TERM_MASS if terms == "both" else 0)
+    out = []
+    for mode, umax in PATCHES:
+        monkeypatch.setenv("MGFE_PATCH", mode)
+        if umax is None:
+            monkeypatch.delenv("MGFE_PATCH_MAX", raising=False)
+        else:
+            monkeypatch.setenv("MGFE_PATCH_MAX", umax)
+        setup(ctx, m, 1)
+        ctx.field_set(L.FIELD_STIFF, k)
+        ctx.assemble(sysid, t, stiff_coeff=L.COEFF_P0, stiff_rule=rs, mass_rule=rm, prec=prec)
+        out.append(ctx.values(sysid, prec=prec))
+    for o in out[1:]:
+        assert np.array_equal(out[0], o)
