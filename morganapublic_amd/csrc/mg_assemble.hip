@@ -237,14 +237,17 @@ __global__ __launch_bounds__(256) void k_elem_factors(
 }
 
 // What the P1 row kernels need to form element factors in place.
-// The vertex patches of a patched P1 segment (RecP1p records): per 64-row
-// block, its vertices (ptr, vid) and each row's own vertex (row_lv); slots is
-// the number of vertices whose coordinates the segment stages per block.
+// The vertex patches of a patched segment: per patch block, its vertices
+// (ptr, vid), each row's own vertex or endpoints (row_lv), the P2 patch words
+// (inc_lv; P1 records hold their patch indices); slots is the number of
+// vertices whose coordinates the segment stages per block.
 struct PatchSrc {
   const int32_t *ptr;
   const int32_t *vid;
-  const uint16_t *row_lv;
+  const uint32_t *row_lv;
+  const uint32_t *inc_lv;
   int slots;
+  int pb0;  // patch block of the segment's first row
 };
 
 struct ElemSrc {
@@ -510,16 +513,17 @@ __device__ __forceinline__ void row_values(const T e[6], T md, T v[NL]) {
 // incidence order, so the result is the staged form's bit for bit.
 template <int NL, int LROW, typename T, bool MASS, bool EXACT, bool ELIM, int ROWS, int LPR,
           typename Rec, bool GACC = false, bool PATCH = false>
-// patched rows: at most 80 VGPRs, so 6 waves per SIMD fit, as many as the
-// LDS of 64-row blocks allows
-__global__ __launch_bounds__(ROWS *LPR, PATCH ? 6 : 1) void k_asm_rows(
+// patched P1 rows: at most 80 VGPRs, so 6 waves per SIMD fit, as many as
+// the LDS of their 64-row blocks allows (P2 blocks stage more entries, and
+// their LDS allows about 3 waves per SIMD for vertex rows, 4 for edge rows)
+__global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? 6 : LROW == 0 ? 3 : 4) void k_asm_rows(
     int32_t row_begin, int32_t row_end, int32_t dummy, EdgePack ep,
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ inc_ptr,
     const int32_t *__restrict__ diagpos, const Rec *__restrict__ rec, const T *__restrict__ geo,
     ElemSrc es, PatchSrc pc, T *__restrict__ vals) {
   static_assert(!GACC || LPR == 1, "global accumulation takes one lane per row");
-  static_assert(!PATCH || (NL == 4 && LROW == 0 && ROWS == kPatchRows && !GACC),
-                "vertex patches: P1 rows in kPatchRows-row staged blocks");
+  static_assert(!PATCH || (ROWS == kPatchRows && !GACC),
+                "vertex patches: kPatchRows-row staged blocks");
   extern __shared__ __align__(16) char lds_raw[];
   constexpr int NT = ROWS * LPR;
   constexpr int GS = geo_stride<T, MASS>();
@@ -545,7 +549,7 @@ __global__ __launch_bounds__(ROWS *LPR, PATCH ? 6 : 1) void k_asm_rows(
   // read the vertex coordinates from the block's patch in LDS, so that a
   // record and a coefficient are the only per-incidence global loads.
   constexpr bool kInl = NL == 4;
-  constexpr bool kChunked = kInl || LROW == 4 || sizeof(T) == 4;
+  constexpr bool kChunked = kInl || PATCH || LROW == 4 || sizeof(T) == 4;
   const int r = r0 + lane / LPR;
   const int rr = kChunked ? min(r, r1 - 1) : r;
   // compact records leave out the row's own column: its position is the
@@ -563,13 +567,16 @@ __global__ __launch_bounds__(ROWS *LPR, PATCH ? 6 : 1) void k_asm_rows(
   // staged entries; npm = the patch's last index (record indices are clamped
   // to it: a row's records always point into its own block's patch)
   double *PX = nullptr;
-  int lvr = 0, npm = 0;
+  uint32_t lvr = 0;
+  int npm = 0;
   // Patched rows: the lane's first KP incidences (records, then their
   // coefficients) are requested before the patch is staged, so that their
   // latency overlaps the staging and the barrier; a block then waits for
-  // about three memory round trips before it computes.
-  constexpr int KP = PATCH ? 6 : 1;
+  // about three memory round trips before it computes.  Vertex rows carry
+  // about 24 incidences, P2 edge rows about 5.
+  constexpr int KP = !PATCH ? 1 : LROW == 0 ? 6 : 3;
   RV Rp[KP];
+  uint32_t lvw[KP];  // P2 patch words
   double ksp[KP], kmp[KP];
   int lastp = 0;
   if constexpr (PATCH) {
@@ -578,13 +585,14 @@ __global__ __launch_bounds__(ROWS *LPR, PATCH ? 6 : 1) void k_asm_rows(
     for (int k = 0; k < KP; ++k) {
       const int j = i + k * LPR;
       Rp[k] = RV::load(rec, j < iend ? j : lastp, self);
+      if constexpr (NL == 10) lvw[k] = pc.inc_lv[j < iend ? j : lastp];
     }
     PX = reinterpret_cast<double *>(lds_raw +
                                     ((((size_t)dummy + 256) * sizeof(T) + 7) & ~(size_t)7));
-    const int pb = r0 / kPatchRows;
+    const int pb = pc.pb0 + (r0 - row_begin) / kPatchRows;
     const int p0 = pc.ptr[pb], np = pc.ptr[pb + 1] - p0;
     npm = max(np - 1, 0);
-    lvr = min((int)pc.row_lv[rr], npm);
+    lvr = pc.row_lv[rr];
     // up to three vertices per lane with their loads in flight together
     constexpr int KV = 3;
     int vid[KV];
@@ -617,11 +625,16 @@ __global__ __launch_bounds__(ROWS *LPR, PATCH ? 6 : 1) void k_asm_rows(
   }
   __syncthreads();
   if (kChunked) asm volatile("" ::"v"(base), "v"(iend), "v"(i), "v"(self));
-  double own[3] = {0.0, 0.0, 0.0};  // the row's vertex (patched rows)
+  // the row's vertex, or its two endpoints (patched rows)
+  double own[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
   if constexpr (PATCH) {
-    own[0] = PX[lvr];
-    own[1] = PX[pc.slots + lvr];
-    own[2] = PX[2 * pc.slots + lvr];
+#pragma unroll
+    for (int h = 0; h < (LROW == 0 ? 1 : 2); ++h) {
+      const int lv = min((int)((lvr >> (16 * h)) & 0xFFFFu), npm);
+      own[h][0] = PX[lv];
+      own[h][1] = PX[pc.slots + lv];
+      own[h][2] = PX[2 * pc.slots + lv];
+    }
   }
   if (r < r1) {
     if (!kChunked) {
@@ -752,7 +765,10 @@ __global__ __launch_bounds__(ROWS *LPR, PATCH ? 6 : 1) void k_asm_rows(
       for (; i < iend; i += KP * LPR) {
         if (!first) {
 #pragma unroll
-          for (int k = 0; k < KP; ++k) Rp[k] = RV::load(rec, at(i + k * LPR), self);
+          for (int k = 0; k < KP; ++k) {
+            Rp[k] = RV::load(rec, at(i + k * LPR), self);
+            if constexpr (NL == 10) lvw[k] = pc.inc_lv[at(i + k * LPR)];
+          }
 #pragma unroll
           for (int k = 0; k < KP; ++k) {
             const int el = Rp[k].el >> 4;
@@ -763,31 +779,54 @@ __global__ __launch_bounds__(ROWS *LPR, PATCH ? 6 : 1) void k_asm_rows(
         first = false;
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
-          double P[4][3];
+          // edge-row lanes past their row's last incidence skip it (whole
+          // waves past it branch over); vertex rows run it masked, as a
+          // branch costs registers there
+          if (LROW == 0 || i + k * LPR < iend) {
+            // relabelled vertex 0 (and 1, edge rows) from the row's, the rest
+            // from the patch
+            constexpr int M0 = LROW == 0 ? 1 : 2;
+            double P[4][3];
+            if constexpr (LROW == 0) {
 #pragma unroll
-          for (int cc = 0; cc < 3; ++cc) P[0][cc] = own[cc];
+              for (int cc = 0; cc < 3; ++cc) P[0][cc] = own[0][cc];
+            } else {
+              const bool sw = (lvw[k] >> 30) & 1u;
 #pragma unroll
-          for (int m = 1; m < 4; ++m) {
-            const int lv = min(Rp[k].lv(m - 1), npm);
-            P[m][0] = PX[lv];
-            P[m][1] = PX[pc.slots + lv];
-            P[m][2] = PX[2 * pc.slots + lv];
+              for (int cc = 0; cc < 3; ++cc) {
+                P[0][cc] = sw ? own[1][cc] : own[0][cc];
+                P[1][cc] = sw ? own[0][cc] : own[1][cc];
+              }
+            }
+#pragma unroll
+            for (int m = M0; m < 4; ++m) {
+              int lv;
+              if constexpr (NL == 4)
+                lv = Rp[k].lv(m - M0);
+              else
+                lv = (int)((lvw[k] >> (10 * (m - M0))) & 0x3FFu);
+              lv = min(lv, npm);
+              P[m][0] = PX[lv];
+              P[m][1] = PX[pc.slots + lv];
+              P[m][2] = PX[2 * pc.slots + lv];
+            }
+            T f[8];
+            edge_factors<T, MASS>(P, ksp[k], kmp[k], f);
+            Slot A;
+            A.r = Rp[k];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) A.e[j] = edge_needed<LROW, EXACT>(j) ? f[j] : (T)0;
+            A.md = MASS ? f[6] : (T)0;
+            add(A, i + k * LPR);
           }
-          T f[8];
-          edge_factors<T, MASS>(P, ksp[k], kmp[k], f);
-          Slot A;
-          A.r = Rp[k];
-#pragma unroll
-          for (int j = 0; j < 6; ++j) A.e[j] = edge_needed<LROW, EXACT>(j) ? f[j] : (T)0;
-          A.md = MASS ? f[6] : (T)0;
-          add(A, i + k * LPR);
         }
       }
     } else if constexpr (kInl) {
-      // A lane takes 3 incidences at once (2: 626 us, 4: 764 us at 128^3):
-      // their records, then vertex ids and coefficients, then the vertex
-      // coordinates, and forms each element's factors with the arithmetic of
-      // k_elem_factors (the values are bit-identical to the factor path).
+      // P1 rows without a patch.  A lane takes 3 incidences at once (2: 626
+      // us, 4: 764 us at 128^3): their records, then vertex ids and
+      // coefficients, then the vertex coordinates in relabelled order, and
+      // forms each element's factors as the patched rows do (the same
+      // values, bit for bit).
       constexpr int KC = 3;
       const int4 *__restrict__ tets4 = reinterpret_cast<const int4 *>(es.tets);
       for (; i < iend; i += KC * LPR) {
@@ -884,9 +923,23 @@ using IC = std::integral_constant<int, V>;
 template <bool V>
 using BC = std::integral_constant<bool, V>;
 
-// Whether an assembly of S in this precision needs the factor kernel: P1
-// rows form the factors in place.
-static bool needs_factors(const System &, int nl, size_t) { return nl != 4; }
+// Whether an assembly of S needs the factor kernel: P1 rows and patched FP64
+// P2 rows form the factors in place.
+static bool needs_factors(const System &S, int nl, size_t tsize) {
+  if (nl == 4) return false;
+  if (tsize == 4) return true;
+  for (const RowSegment &sg : S.segs)
+    if (!sg.patch) return true;
+  return false;
+}
+
+// Whether a segment's rows use its vertex patch: P1 rows in both precisions,
+// P2 rows in FP64 (FP32 P2 rows read the factor records: measured 154 us per
+// 64^3 P2 FP32 assembly against 173 us in place, where the edge rows repeat
+// the element geometry six times per element)
+static bool use_patch(const RowSegment &sg, int nl, size_t tsize) {
+  return sg.patch > 0 && (nl == 4 || tsize == 8);
+}
 
 // geo_buf: where the element factor records go (nullptr: the context's own
 // array, grown as needed)
@@ -927,9 +980,10 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
     const int nblk = (sg.row_end - sg.row_begin + sg.rows - 1) / sg.rows;
     // staged entries, the 256-slot region of eliminated columns, then (at
     // 8-byte alignment) the vertex patch coordinates
+    const bool patched = use_patch(sg, NL, sizeof(T));
     size_t lds = sg.gacc ? 0 : sizeof(T) * (size_t)(sg.lds_slots + 256);
-    if (sg.patch) lds = ((lds + 7) & ~(size_t)7) + 24 * (size_t)sg.patch;
-    const PatchSrc pc{S.patch_ptr, S.patch_vid, S.row_lv, sg.patch};
+    if (patched) lds = ((lds + 7) & ~(size_t)7) + 24 * (size_t)sg.patch;
+    const PatchSrc pc{S.patch_ptr, S.patch_vid, S.row_lv, S.inc_lv, sg.patch, sg.pb0};
     auto go = [&](auto lr, auto ms, auto ex, auto rw, auto lp, auto ga, auto pt) {
       if (elim)
         k_asm_rows<NL, decltype(lr)::value, T, decltype(ms)::value, decltype(ex)::value, true,
@@ -957,8 +1011,9 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
       }
       if constexpr (decltype(lr)::value == 0) {
         const int lpr = sizeof(T) == 4 ? sg.lpr32 : sg.lpr;
-        if constexpr (std::is_same<Rec, RecP1p>::value) {
-          if (sg.patch) {
+        if constexpr (std::is_same<Rec, RecP1p>::value ||
+                      (std::is_same<Rec, RecP2c>::value && sizeof(T) == 8)) {
+          if (patched) {
             const BC<true> pt{};
             if (lpr == 4)
               go(lr, ms, ex, IC<kPatchRows>{}, IC<4>{}, st, pt);
@@ -976,6 +1031,14 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
         else
           go(lr, ms, ex, IC<64>{}, IC<1>{}, st, st);
       } else {
+        if constexpr (std::is_same<Rec, RecP2c>::value && sizeof(T) == 8) {
+          if (patched) {
+            const BC<true> pt{};
+            sg.lpr == 2 ? go(lr, ms, ex, IC<kPatchRows>{}, IC<2>{}, st, pt)
+                        : go(lr, ms, ex, IC<kPatchRows>{}, IC<1>{}, st, pt);
+            return;
+          }
+        }
         if (sg.rows == 256)
           sg.lpr == 2 ? go(lr, ms, ex, IC<256>{}, IC<2>{}, st, st)
                       : go(lr, ms, ex, IC<256>{}, IC<1>{}, st, st);

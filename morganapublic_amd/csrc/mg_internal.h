@@ -56,14 +56,21 @@ struct alignas(4) RecP1p {
 };
 enum RecKind { kRecP1 = 1, kRecP1w, kRecP2, kRecP2c, kRecP2w, kRecP1p };
 
-// Vertex patches of P1 row blocks: the sorted distinct vertices of the
-// elements incident to kPatchRows consecutive rows, whose coordinates a row
-// block stages in LDS.  A block gets a patch when its vertex references fit
-// the LDS sort (kPatchKeys), it has at most kPatchMax distinct vertices and
-// its entries and coordinates fit the assembly's LDS budget.
+// Vertex patches of row blocks: the sorted distinct vertices of the
+// elements incident to kPatchRows consecutive rows (vertex-dof and P2
+// edge-dof rows in separate blocks), whose coordinates a row block stages in
+// LDS.  A block gets a patch when its vertex references fit the LDS sort
+// (kPatchKeys), it has at most kPatchMax distinct vertices and its entries
+// and coordinates fit the assembly's LDS budget.
 constexpr int kPatchRows = 64;
-constexpr int kPatchMax = 2048;
+constexpr int kPatchMax = 2048;    // P1: 12-bit indices in the records
+constexpr int kPatchMaxP2 = 1023;  // P2: 10-bit indices in the patch words
 constexpr int kPatchKeys = 8192;
+// LDS bytes of a patched FP64 row block: its entries and the 256-slot
+// region of eliminated columns, then the patch coordinates (3 doubles each)
+inline size_t patch_lds(int64_t entries, int64_t verts) {
+  return (size_t)(entries + 256) * 8 + 24 * (size_t)verts;
+}
 
 // Rows of one assembly launch: blocks of `rows` rows, each staging its CSR
 // entries in LDS; lrow is the local basis every incidence of these rows is relabelled to
@@ -76,8 +83,8 @@ struct RowSegment {
   // values array itself, one lane per row (no staging)
   int32_t gacc;
   // > 0: every block has a vertex patch of at most this many vertices, staged
-  // in LDS after the entries (P1 rows)
-  int32_t patch;
+  // in LDS after the entries; pb0 = the patch block of the first row
+  int32_t patch, pb0;
 };
 
 constexpr int kSpmvRows = 64;  // rows per SpMV block (128 for short rows)
@@ -123,12 +130,19 @@ struct System {
   int rec_words = 0;           // int32 words per record: 2 (RecP1), 3 (RecP1w, RecP1p, RecP2c), 4 (RecP2), 6 (RecP2w)
   int rec_kind = 0;            // RecKind
   std::vector<RowSegment> segs;
-  // vertex patches (RecP1p systems): block b = rows [64 b, 64 b + 64) has
-  // vertices patch_vid[patch_ptr[b] .. patch_ptr[b + 1]) (none: equal
-  // pointers); row_lv[r] is the index of row r's own vertex in its patch
+  // vertex patches (RecP1p systems, and RecP2c systems with inc_lv): patch
+  // block b (kPatchRows rows: vertex-dof rows from row 0, then edge-dof rows
+  // from the first edge row) has vertices patch_vid[patch_ptr[b] ..
+  // patch_ptr[b + 1]) (none: equal pointers); row_lv[r] holds the patch
+  // index of row r's vertex (vertex rows) or endpoints (edge rows: low and
+  // high 16 bits).  P2 patch word of incidence i, inc_lv[i]: the 10-bit patch
+  // indices of relabelled vertices 1-3 (vertex rows) or 2-3 (edge rows, with
+  // bit 30 set when relabelled vertex 0 is the row's second endpoint); P1
+  // records carry theirs.
   int32_t *patch_ptr = nullptr;
   int32_t *patch_vid = nullptr;
-  uint16_t *row_lv = nullptr;
+  uint32_t *row_lv = nullptr;
+  uint32_t *inc_lv = nullptr;
   // SpMV row blocks: each holds <= kSpmvNnz entries and <= kSpmvRows rows
   BlkDesc *sdesc = nullptr;
   int32_t n_sblk = 0;

@@ -469,13 +469,16 @@ __global__ void k_inc_records_p1p(int64_t n_inc, const uint64_t *__restrict__ ke
   rec[i] = RecP1p{v, w, 0u};
 }
 
-// Vertex patch of row block b (rows kPatchRows b ..): the vertices of the
-// elements of the block's incidences, sorted in LDS (bitonic) and made
-// distinct.  FILL false: cnt[b] = the number of distinct vertices (-1: more
-// than kPatchKeys references).  FILL true, for blocks given a patch
-// (pptr[b + 1] > pptr[b]): the vertices go to vid[pptr[b] ..], every record of
-// the block gets the patch indices of its relabelled element's vertices 1-3,
-// and row_lv[r] the index of row r's own vertex (relabelled vertex 0).
+// Vertex patch of patch block b: the vertices of the elements of the block's
+// incidences, sorted in LDS (bitonic) and made distinct.  Blocks b < nvb hold
+// vertex-dof rows from row 0, the others edge-dof rows from row nvrows.
+// FILL false: cnt[b] = the number of distinct vertices (-1: more than
+// kPatchKeys references).  FILL true, for blocks given a patch
+// (pptr[b + 1] > pptr[b]): the vertices go to vid[pptr[b] ..]; a lane per row
+// sets row_lv[r] and the patch indices of each of the row's incidences (in
+// its P1 record, in inc_lv for P2): of relabelled vertices 1-3 (vertex rows)
+// or 2-3 and the orientation bit (P2 edge rows, whose endpoints are
+// relabelled vertices 0 and 1 of the row's first record).
 __device__ __forceinline__ int patch_find(const int32_t *u, int n, int32_t v) {
   int lo = 0, hi = n;  // first index with u[index] >= v
   while (lo < hi) {
@@ -488,18 +491,32 @@ __device__ __forceinline__ int patch_find(const int32_t *u, int n, int32_t v) {
   return lo;
 }
 
-template <bool FILL>
-__global__ __launch_bounds__(256) void k_patch(int64_t nrows, const int32_t *__restrict__ inc_ptr,
-                                               RecP1p *__restrict__ rec,
+__device__ __forceinline__ void patch_block_rows(int b, int64_t nvrows, int nvb, int64_t nrows,
+                                                 int64_t &r0, int64_t &r1) {
+  if (b < nvb) {
+    r0 = (int64_t)b * kPatchRows;
+    r1 = min<int64_t>(r0 + kPatchRows, nvrows);
+  } else {
+    r0 = nvrows + (int64_t)(b - nvb) * kPatchRows;
+    r1 = min<int64_t>(r0 + kPatchRows, nrows);
+  }
+}
+
+template <bool FILL, typename Rec>
+__global__ __launch_bounds__(256) void k_patch(int64_t nvrows, int nvb, int64_t nrows,
+                                               const int32_t *__restrict__ inc_ptr,
+                                               Rec *__restrict__ rec,
                                                const int32_t *__restrict__ tets,
                                                const int32_t *__restrict__ pptr,
                                                int32_t *__restrict__ out,
-                                               uint16_t *__restrict__ row_lv) {
+                                               uint32_t *__restrict__ row_lv,
+                                               uint32_t *__restrict__ inc_lv) {
   __shared__ int32_t key[kPatchKeys];
   __shared__ int32_t uniq[kPatchMax];
   __shared__ int32_t part[257];
   const int b = blockIdx.x, t = threadIdx.x;
-  const int64_t r0 = (int64_t)b * kPatchRows, r1 = min<int64_t>(r0 + kPatchRows, nrows);
+  int64_t r0, r1;
+  patch_block_rows(b, nvrows, nvb, nrows, r0, r1);
   const int i0 = inc_ptr[r0], i1 = inc_ptr[r1];
   const int n4 = 4 * (i1 - i0);
   if (FILL) {
@@ -553,42 +570,63 @@ __global__ __launch_bounds__(256) void k_patch(int64_t nrows, const int32_t *__r
       ++u;
     }
   __syncthreads();
-  for (int j = i0 + t; j < i1; j += 256) {
-    RecP1p q = rec[j];
-    const int e = q.el >> 4, l = q.el & 15;
-    uint32_t lv[3];
-    for (int m = 1; m < 4; ++m)
-      lv[m - 1] = (uint32_t)patch_find(uniq, U, tets[4 * (size_t)e + c_sigma[l][m]]);
-    q.lo = (q.lo & 0xFFFFFFu) | (lv[0] << 24);
-    q.hi = (lv[0] >> 8) | (lv[1] << 4) | (lv[2] << 16);
-    rec[j] = q;
-  }
+  auto find = [&](int64_t e, int l, int m) {
+    return (uint32_t)patch_find(uniq, U, tets[4 * e + c_sigma[l][m]]);
+  };
   for (int64_t r = r0 + t; r < r1; r += 256) {
-    const int j = inc_ptr[r];
-    int own = 0;
-    if (j < inc_ptr[r + 1]) {
-      const int32_t el = rec[j].el;
-      own = patch_find(uniq, U, tets[4 * (size_t)(el >> 4) + (el & 15)]);
+    const int j0 = inc_ptr[r], j1 = inc_ptr[r + 1];
+    uint32_t own = 0;
+    int32_t va = -1;  // the row's first endpoint (edge rows)
+    if (j0 < j1) {
+      const int32_t el = rec[j0].el;
+      const int64_t e = el >> 4;
+      const int l = el & 15;
+      own = find(e, l, 0);
+      if (l >= 4) {
+        own |= find(e, l, 1) << 16;
+        va = tets[4 * e + c_sigma[l][0]];
+      }
     }
-    row_lv[r] = (uint16_t)own;
+    row_lv[r] = own;
+    for (int j = j0; j < j1; ++j) {
+      Rec q = rec[j];
+      const int64_t e = q.el >> 4;
+      const int l = q.el & 15;
+      if constexpr (std::is_same<Rec, RecP1p>::value) {
+        const uint32_t v0 = find(e, l, 1), v1 = find(e, l, 2), v2 = find(e, l, 3);
+        q.lo = (q.lo & 0xFFFFFFu) | (v0 << 24);
+        q.hi = (v0 >> 8) | (v1 << 4) | (v2 << 16);
+        rec[j] = q;
+      } else if (l < 4) {
+        inc_lv[j] = find(e, l, 1) | (find(e, l, 2) << 10) | (find(e, l, 3) << 20);
+      } else {
+        const uint32_t sw = tets[4 * e + c_sigma[l][0]] == va ? 0u : 1u;
+        inc_lv[j] = find(e, l, 2) | (find(e, l, 3) << 10) | (sw << 30);
+      }
+    }
   }
 }
 
-// Vertex patches of a RecP1p system; returns the patch size of every row
-// block (0: no patch).  A block gets one when its distinct vertices fit the
-// index form and its staged entries and coordinates each take at most half
-// of the 64 KiB the assembly stages per block (MGFE_PATCH=2: patched
-// records, no patches; MGFE_PATCH_MAX: a lower vertex limit, which mixes
-// blocks with and without patches on small meshes).
-static std::vector<int32_t> build_patches(Ctx *c, System &S, const std::vector<int32_t> &rp) {
-  const int64_t nb = (S.nrows + kPatchRows - 1) / kPatchRows;
+// Vertex patches of a RecP1p / RecP2c system (vertex rows [0, nvrows), P2
+// edge rows after them); returns the patch size of every patch block (0: no
+// patch).  A block gets one when its distinct vertices fit the index form
+// and its staged entries and coordinates fit the 64 KiB an assembly block
+// stages (MGFE_PATCH=2: patched records, no patches; MGFE_PATCH_MAX: a lower
+// vertex limit, which mixes blocks with and without patches on small meshes).
+template <typename Rec>
+static std::vector<int32_t> build_patches(Ctx *c, System &S, const std::vector<int32_t> &rp,
+                                          int64_t nvrows) {
+  const int nvb = (int)((nvrows + kPatchRows - 1) / kPatchRows);
+  const int64_t nb = nvb + (S.nrows - nvrows + kPatchRows - 1) / kPatchRows;
   std::vector<int32_t> hc(nb, 0), pp(nb + 1, 0);
   const bool on = env_int("MGFE_PATCH", 1) == 1;
-  const int umax = std::min(kPatchMax, std::max(1, env_int("MGFE_PATCH_MAX", kPatchMax)));
-  if (on) {
+  const int lim = std::is_same<Rec, RecP1p>::value ? kPatchMax : kPatchMaxP2;
+  const int umax = std::min(lim, std::max(1, env_int("MGFE_PATCH_MAX", lim)));
+  if (on && nb > 0) {
     int32_t *cnt = dalloc<int32_t>(c, nb);
-    k_patch<false><<<(unsigned)nb, 256, 0, c->stream>>>(S.nrows, S.inc_ptr, (RecP1p *)S.inc_rec,
-                                                        c->tets, nullptr, cnt, nullptr);
+    k_patch<false, Rec><<<(unsigned)nb, 256, 0, c->stream>>>(nvrows, nvb, S.nrows, S.inc_ptr,
+                                                             (Rec *)S.inc_rec, c->tets, nullptr,
+                                                             cnt, nullptr, nullptr);
     MG_HIP(hipGetLastError());
     MG_HIP(hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * nb, hipMemcpyDeviceToHost,
                           c->stream));
@@ -596,23 +634,28 @@ static std::vector<int32_t> build_patches(Ctx *c, System &S, const std::vector<i
     dfree(c, cnt);
   }
   for (int64_t b = 0; b < nb; ++b) {
-    const int64_t r0 = b * kPatchRows, r1 = std::min<int64_t>(r0 + kPatchRows, S.nrows);
+    const int64_t r0 = b < nvb ? b * kPatchRows : nvrows + (b - nvb) * kPatchRows;
+    const int64_t r1 = std::min<int64_t>(r0 + kPatchRows, b < nvb ? nvrows : S.nrows);
     const int ent = rp[r1] - rp[r0];
     const int u = hc[b];
-    const bool ok = u > 0 && u <= umax && (ent + 256) * 8 <= 32 * 1024 && u * 24 <= 32 * 1024;
+    const bool ok = u > 0 && u <= umax && patch_lds(ent, u) <= 64 * 1024;
     hc[b] = ok ? u : 0;
     pp[b + 1] = pp[b] + hc[b];
   }
   S.patch_ptr = dalloc<int32_t>(c, nb + 1);
   S.patch_vid = dalloc<int32_t>(c, std::max<int64_t>(pp[nb], 1));
-  S.row_lv = dalloc<uint16_t>(c, S.nrows);
+  S.row_lv = dalloc<uint32_t>(c, S.nrows);
   MG_HIP(hipMemcpyAsync(S.patch_ptr, pp.data(), sizeof(int32_t) * (nb + 1), hipMemcpyHostToDevice,
                         c->stream));
-  MG_HIP(hipMemsetAsync(S.row_lv, 0, sizeof(uint16_t) * S.nrows, c->stream));
+  MG_HIP(hipMemsetAsync(S.row_lv, 0, sizeof(uint32_t) * S.nrows, c->stream));
   if (pp[nb] > 0) {
-    k_patch<true><<<(unsigned)nb, 256, 0, c->stream>>>(S.nrows, S.inc_ptr, (RecP1p *)S.inc_rec,
-                                                       c->tets, S.patch_ptr, S.patch_vid,
-                                                       S.row_lv);
+    if (!std::is_same<Rec, RecP1p>::value) {
+      S.inc_lv = dalloc<uint32_t>(c, S.n_inc);
+      MG_HIP(hipMemsetAsync(S.inc_lv, 0, sizeof(uint32_t) * S.n_inc, c->stream));
+    }
+    k_patch<true, Rec><<<(unsigned)nb, 256, 0, c->stream>>>(nvrows, nvb, S.nrows, S.inc_ptr,
+                                                            (Rec *)S.inc_rec, c->tets, S.patch_ptr,
+                                                            S.patch_vid, S.row_lv, S.inc_lv);
     MG_HIP(hipGetLastError());
   }
   MG_HIP(hipStreamSynchronize(c->stream));
@@ -624,8 +667,9 @@ static std::vector<int32_t> build_patches(Ctx *c, System &S, const std::vector<i
 // that takes eliminated columns, in FP64, within 64 KiB
 static bool block_fits(int entries) { return (entries + 256) * 8 <= 64 * 1024; }
 
-// psz: the vertex patch size of every kPatchRows-row block (P1 records with
-// patches), nullptr otherwise
+// psz: the vertex patch size of every patch block (patched records: vertex
+// rows from row 0, then P2 edge rows from nvrows, in kPatchRows-row blocks),
+// nullptr otherwise
 static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvrows, int nl,
                           const std::vector<int32_t> *psz) {
   S.segs.clear();
@@ -634,7 +678,7 @@ static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvr
     int mx = 0;
     for (int64_t r = a; r < b; r += rows) mx = std::max(mx, rp[std::min<int64_t>(r + rows, b)] - rp[r]);
     S.segs.push_back(RowSegment{(int32_t)a, (int32_t)b, gacc ? 0 : mx, lrow, rows,
-                                gacc ? 1 : lpr, gacc ? 1 : lpr32, gacc ? 1 : 0, 0});
+                                gacc ? 1 : lpr, gacc ? 1 : lpr32, gacc ? 1 : 0, 0, 0});
   };
   auto add = [&](int64_t a, int64_t b, int lrow, int rows, int lpr, int lpr32) {
     if (b <= a) return;
@@ -675,37 +719,49 @@ static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvr
             er = env_int("MGFE_EDGE_ROWS", 128), el = env_int("MGFE_EDGE_LPR", 2);
   const int vlpr = pick(vl, 1, 2, 4) ? vl : 4;
   const int vlpr32 = pick(vl32, 1, 2, 4) ? vl32 : (nl == 10 ? 2 : 4);
+  const int erows = pick(er, 64, 128, 256) ? er : 128;
+  const int elpr = (el == 1 || el == 2) ? el : 2;
   if (psz) {
-    // P1 rows in kPatchRows-row blocks, segments split where a block's kind
-    // changes: staged (0), accumulated in place (1), staged with a patch (2)
-    auto kind = [&](int64_t r) {
-      if (!block_fits(rp[std::min<int64_t>(r + kPatchRows, nvrows)] - rp[r])) return 1;
-      return (*psz)[r / kPatchRows] > 0 ? 2 : 0;
-    };
-    auto flush = [&](int64_t a, int64_t b, int k) {
-      push(a, b, 0, kPatchRows, vlpr, vlpr32, k == 1);
-      if (k == 2) {
-        int mp = 0;
-        for (int64_t r = a; r < b; r += kPatchRows) mp = std::max(mp, (*psz)[r / kPatchRows]);
-        S.segs.back().patch = mp;
+    // kPatchRows-row blocks, segments split where a block's kind changes
+    // (staged: 0, accumulated in place: 1, staged with a patch: 2) and
+    // where a patched segment's largest entries and patch would together
+    // pass the 64 KiB LDS budget
+    auto region = [&](int64_t a, int64_t b, int lrow, int pb0, int lpr, int lpr32) {
+      auto kind = [&](int64_t r) {
+        if (!block_fits(rp[std::min<int64_t>(r + kPatchRows, b)] - rp[r])) return 1;
+        return (*psz)[pb0 + (r - a) / kPatchRows] > 0 ? 2 : 0;
+      };
+      int64_t run = a;
+      int rk = -1, me = 0, mp = 0;
+      auto flush = [&](int64_t x) {
+        if (x <= run) return;
+        push(run, x, lrow, kPatchRows, lpr, lpr32, rk == 1);
+        if (rk == 2) {
+          S.segs.back().patch = mp;
+          S.segs.back().pb0 = pb0 + (int)((run - a) / kPatchRows);
+        }
+        run = x;
+      };
+      for (int64_t r = a; r < b; r += kPatchRows) {
+        const int k = kind(r);
+        const int ent = rp[std::min<int64_t>(r + kPatchRows, b)] - rp[r];
+        const int u = (*psz)[pb0 + (r - a) / kPatchRows];
+        if (k != rk || (k == 2 && patch_lds(std::max(me, ent), std::max(mp, u)) > 64 * 1024)) {
+          flush(r);
+          rk = k;
+          me = mp = 0;
+        }
+        me = std::max(me, ent);
+        mp = std::max(mp, u);
       }
+      flush(b);
     };
-    int64_t run = 0;
-    int rk = nvrows > 0 ? kind(0) : 0;
-    for (int64_t r = 0; r < nvrows; r += kPatchRows) {
-      const int k = kind(r);
-      if (k != rk) {
-        flush(run, r, rk);
-        run = r;
-        rk = k;
-      }
-    }
-    flush(run, nvrows, rk);
+    const int nvb = (int)((nvrows + kPatchRows - 1) / kPatchRows);
+    region(0, nvrows, 0, 0, vlpr, vlpr32);
+    region(nvrows, S.nrows, 4, nvb, elpr, elpr);
     return;
   }
   add(0, nvrows, 0, 64, vlpr, vlpr32);
-  const int erows = pick(er, 64, 128, 256) ? er : 128;
-  const int elpr = (el == 1 || el == 2) ? el : 2;
   add(nvrows, S.nrows, 4, erows, elpr, elpr);
 }
 
@@ -815,9 +871,9 @@ void build_pattern(Ctx *c, int sys) {
   // positions up to 254, uint16 positions beyond (up to 65534)
   MG_CHECK(maxnnz <= 65534, MG_ERR_SIZE, "pattern: a row has more than 65534 entries");
   const bool compact = nl == 10 && maxnnz <= 127 && env_int("MGFE_REC16", 0) == 0;
-  // P1 rows up to 254 entries: patched records (MGFE_PATCH=0: the 8-byte
-  // form without patches)
-  const bool patched = nl == 4 && maxnnz <= 254 && env_int("MGFE_PATCH", 1) != 0;
+  // patched records: P1 rows up to 254 entries, P2 rows up to 127
+  // (MGFE_PATCH=0: the forms without patches)
+  const bool patched = (nl == 4 ? maxnnz <= 254 : compact) && env_int("MGFE_PATCH", 1) != 0;
   const bool wide = maxnnz > 254;
   const int gi = blocks_for(S.n_inc, 256);
   if (compact) {
@@ -865,8 +921,10 @@ void build_pattern(Ctx *c, int sys) {
   MG_CHECK(hb == 0, MG_ERR_STATE, "pattern: a record position does not fit its form");
   const int64_t nvrows =
       nl == 4 ? nrows : (sys == MG_SYS_FULL ? c->nV : c->nfree_vertex);
-  if (S.rec_kind == kRecP1p) {
-    const std::vector<int32_t> psz = build_patches(c, S, rp);
+  if (S.rec_kind == kRecP1p || (S.rec_kind == kRecP2c && patched)) {
+    const std::vector<int32_t> psz = S.rec_kind == kRecP1p
+                                         ? build_patches<RecP1p>(c, S, rp, nvrows)
+                                         : build_patches<RecP2c>(c, S, rp, nvrows);
     make_segments(S, rp, nvrows, nl, &psz);
   } else {
     make_segments(S, rp, nvrows, nl, nullptr);

@@ -180,16 +180,27 @@ SHAPES = [("1", "64", "1", "0"), ("2", "128", "2", "0"), ("4", "256", "1", "0"),
           ("4", "256", "2", "0"), ("4", "256", "2", "1"), ("1", "64", "1", "1")]
 
 
+def _row_scaled_diff(ctx, sysid, a, b):
+    rp, _ = ctx.pattern(sysid)
+    rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+    scale = np.zeros(rp.size - 1)
+    np.maximum.at(scale, rows, np.abs(a.astype(np.float64)))
+    scale[scale == 0] = 1.0
+    return float((np.abs(a.astype(np.float64) - b) / scale[rows]).max())
+
+
 @pytest.mark.parametrize("order,rs", [(1, 1), (2, 11), (2, 15)])
 @pytest.mark.parametrize("prec", [L.PREC_F64, L.PREC_F32])
 @pytest.mark.parametrize("sysid", [L.SYS_FULL, L.SYS_INTERIOR])
 def test_assembly_launch_shapes_bit_identical(ctx, monkeypatch, order, rs, prec, sysid):
     # lanes per row, rows per workgroup and the record form change the
     # schedule and the bytes read, never the order in which a row receives
-    # its incidences
+    # its incidences; FP64 P2 rows with vertex patches (12-byte form off)
+    # form the element factors in the row's relabelled frame, the 16-byte
+    # form reads them from the factor kernel: the two agree to rounding
     m = W.kuhn_cube(7, jitter=0.2, seed=5)
     k = np.exp(W.stream_normal(0, W.STREAM_K, m.nT))
-    out = []
+    out = {}
     for vl, er, el, r16 in SHAPES:
         monkeypatch.setenv("MGFE_VERTEX_LPR", vl)
         monkeypatch.setenv("MGFE_VERTEX_LPR32", vl)
@@ -200,8 +211,67 @@ def test_assembly_launch_shapes_bit_identical(ctx, monkeypatch, order, rs, prec,
         ctx.field_set(L.FIELD_STIFF, k)
         ctx.assemble(sysid, L.TERM_STIFF | L.TERM_MASS, stiff_coeff=L.COEFF_P0,
                      stiff_rule=rs, mass_rule=15, prec=prec)
+        out.setdefault(r16, []).append(ctx.values(sysid, prec=prec))
+    for group in out.values():
+        assert all(np.array_equal(group[0], o) for o in group[1:])
+    if order == 1 or prec == L.PREC_F32:
+        assert np.array_equal(out["0"][0], out["1"][0])
+    else:
+        assert _row_scaled_diff(ctx, sysid, out["0"][0], out["1"][0]) <= 1e-14
+
+
+# record forms: (MGFE_PATCH, MGFE_PATCH_MAX): forms without patches;
+# patched records without patches; patches everywhere; patches on some row
+# blocks only
+PATCHES = [("0", None), ("2", None), ("1", None), ("1", "90"), ("1", "160"), ("1", "260")]
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("terms,rs,rm", [("stiff", 1, 4), ("both", 1, 4), ("both", 4, 1)])
+@pytest.mark.parametrize("prec", [L.PREC_F64, L.PREC_F32])
+@pytest.mark.parametrize("sysid", [L.SYS_FULL, L.SYS_INTERIOR])
+def test_vertex_patches(ctx, monkeypatch, order, terms, rs, rm, prec, sysid):
+    # a row block's element vertices staged in LDS (vertex patch) instead of
+    # gathered per incidence.  P1 rows form the factors in the relabelled
+    # frame with or without a patch: every form gives the same bits.  FP64
+    # P2 rows with a patch form them in the relabelled frame, P2 rows without
+    # one (and FP32 P2 rows) read the factor kernel's: equal bits between
+    # the forms without patches, agreement to rounding across, and the same
+    # bits for every launch shape of a form
+    m = W.kuhn_cube(13 if order == 1 else 6, jitter=0.2, seed=21)
+    k = np.exp(W.stream_normal(0, W.STREAM_K, m.nT))
+    t = L.TERM_STIFF | (L.TERM_MASS if terms == "both" else 0)
+    if order == 2:  # exact rules, or both inexact (table form)
+        rs, rm = (11, 15) if rs == 1 else (1, 4)
+    out = []
+    for mode, umax in PATCHES:
+        monkeypatch.setenv("MGFE_PATCH", mode)
+        if umax is None:
+            monkeypatch.delenv("MGFE_PATCH_MAX", raising=False)
+        else:
+            monkeypatch.setenv("MGFE_PATCH_MAX", umax)
+        setup(ctx, m, order)
+        ctx.field_set(L.FIELD_STIFF, k)
+        ctx.assemble(sysid, t, stiff_coeff=L.COEFF_P0, stiff_rule=rs, mass_rule=rm, prec=prec)
         out.append(ctx.values(sysid, prec=prec))
-    assert all(np.array_equal(out[0], o) for o in out[1:])
+        if mode == "1" and umax is None:
+            for vl in ("1", "2"):
+                monkeypatch.setenv("MGFE_VERTEX_LPR", vl)
+                monkeypatch.setenv("MGFE_VERTEX_LPR32", vl)
+                monkeypatch.setenv("MGFE_EDGE_LPR", vl)
+                setup(ctx, m, order)
+                ctx.field_set(L.FIELD_STIFF, k)
+                ctx.assemble(sysid, t, stiff_coeff=L.COEFF_P0, stiff_rule=rs, mass_rule=rm,
+                             prec=prec)
+                assert np.array_equal(out[-1], ctx.values(sysid, prec=prec))
+            for var in ("MGFE_VERTEX_LPR", "MGFE_VERTEX_LPR32", "MGFE_EDGE_LPR"):
+                monkeypatch.delenv(var, raising=False)
+    assert np.array_equal(out[0], out[1])
+    for o in out[2:]:
+        if order == 1 or prec == L.PREC_F32:
+            assert np.array_equal(out[0], o)
+        else:
+            assert _row_scaled_diff(ctx, sysid, out[0], o) <= 1e-14
 
 
 @pytest.mark.parametrize("order", [1, 2])
@@ -516,32 +586,3 @@ def test_pcg_device_unaligned_vectors(ctx, mode):
         assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) <= 1.5e-10
     assert np.linalg.norm(out[0] - out[1]) <= 1e-8 * np.linalg.norm(out[0])
 
-
-# P1 record forms: (MGFE_PATCH, MGFE_PATCH_MAX): 8-byte records; patched
-# records without patches; patches everywhere; patches on some row blocks
-PATCHES = [("0", None), ("2", None), ("1", None), ("1", "90"), ("1", "160"), ("1", "260")]
-
-
-@pytest.mark.parametrize("terms,rs,rm", [("stiff", 1, 4), ("both", 1, 4), ("both", 4, 1)])
-@pytest.mark.parametrize("prec", [L.PREC_F64, L.PREC_F32])
-@pytest.mark.parametrize("sysid", [L.SYS_FULL, L.SYS_INTERIOR])
-def test_p1_vertex_patches_bit_identical(ctx, monkeypatch, terms, rs, rm, prec, sysid):
-    # a row block's element vertices staged in LDS (vertex patch) instead of
-    # gathered per incidence: the same factors from the same coordinates in
-    # the same order, for blocks with and without a patch in one launch
-    m = W.kuhn_cube(13, jitter=0.2, seed=21)
-    k = np.exp(W.stream_normal(0, W.STREAM_K, m.nT))
-    t = L.TERM_STIFF | (L.TERM_MASS if terms == "both" else 0)
-    out = []
-    for mode, umax in PATCHES:
-        monkeypatch.setenv("MGFE_PATCH", mode)
-        if umax is None:
-            monkeypatch.delenv("MGFE_PATCH_MAX", raising=False)
-        else:
-            monkeypatch.setenv("MGFE_PATCH_MAX", umax)
-        setup(ctx, m, 1)
-        ctx.field_set(L.FIELD_STIFF, k)
-        ctx.assemble(sysid, t, stiff_coeff=L.COEFF_P0, stiff_rule=rs, mass_rule=rm, prec=prec)
-        out.append(ctx.values(sysid, prec=prec))
-    for o in out[1:]:
-        assert np.array_equal(out[0], o)

@@ -60,10 +60,13 @@ def test_noalloc_matches_assemble_on_two_calls(order):
                         assert np.array_equal(got, want), (n, sysid, prec, terms)
 
 
-def test_noalloc_rejects_records_past_its_scratch():
-    # P2 with a mass term in FP64 takes 64 bytes per element: 72^3 Kuhn
-    # tetrahedra (2,239,488) need 143.3 MB, past the 128 MiB array; the
-    # allocating entry point still assembles it
+def test_noalloc_rejects_records_past_its_scratch(monkeypatch):
+    # Without vertex patches, FP64 P2 rows read per-element factor records:
+    # with a mass term 64 bytes per element, so 72^3 Kuhn tetrahedra
+    # (2,239,488) need 143.3 MB, past the 128 MiB array; the allocating entry
+    # point still assembles it.  With patches FP64 P2 rows form the factors
+    # in place and need no records.
+    monkeypatch.setenv("MGFE_PATCH", "0")
     with L.Context(0) as ctx:
         m = W.kuhn_cube(72)
         ctx.mesh_set(m.xyz, m.tets, m.btri, m.bgeo)
@@ -77,3 +80,13 @@ def test_noalloc_rejects_records_past_its_scratch():
         rc = ctx._L.mg_assemble_noalloc(ctx.h, L.SYS_FULL, C.byref(op), L.PREC_F32)
         assert rc == L.MG_OK
         assert ctx._L.mg_assemble(ctx.h, L.SYS_FULL, C.byref(op), L.PREC_F64) == L.MG_OK
+        monkeypatch.setenv("MGFE_PATCH", "1")
+        ctx.mesh_set(m.xyz, m.tets, m.btri, m.bgeo)  # releases the pattern
+        ctx.dofmap_build(2)
+        ctx.pattern_build(L.SYS_FULL)
+        assert ctx._L.mg_assemble(ctx.h, L.SYS_FULL, C.byref(op), L.PREC_F64) == L.MG_OK
+        want = ctx.values(L.SYS_FULL)
+        ctx.assemble(L.SYS_FULL, L.TERM_MASS, mass_rule=4)
+        rc = ctx._L.mg_assemble_noalloc(ctx.h, L.SYS_FULL, C.byref(op), L.PREC_F64)
+        assert rc == L.MG_OK, ctx._L.mg_last_error(ctx.h)
+        assert np.array_equal(ctx.values(L.SYS_FULL), want)
