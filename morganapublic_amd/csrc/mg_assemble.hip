@@ -392,6 +392,36 @@ struct RecP2cView {
     return (int)(v & 0x7Fu);
   }
 };
+// patched P2 edge rows: 5-bit positions, the row's own column (basis LROW)
+// is its diagonal, the patch word is part of the record
+template <int LROW>
+struct RecView<RecP2e, LROW> {
+  static constexpr int kElim = 31;
+  int32_t el;
+  uint32_t lo, hi;
+  int self;
+  __device__ __forceinline__ int pos(int m) const {
+    if (m == LROW) return self;
+    const int k = m < LROW ? m : m - 1;  // packed index, compile-time
+    const int b = 5 * k;
+    const uint32_t v = b >= 32 ? (hi >> (b - 32)) : (b > 27 ? ((lo >> b) | (hi << (32 - b))) : (lo >> b));
+    return (int)(v & 31u);
+  }
+  // in the layout of a separate patch word: the 10-bit fields of relabelled
+  // vertices 2 and 3, the orientation in bit 30
+  __device__ __forceinline__ uint32_t patch_word() const {
+    return ((hi >> 13) & 511u) | (((hi >> 22) & 511u) << 10) | ((hi >> 31) << 30);
+  }
+  __device__ __forceinline__ static RecView load(const RecP2e *__restrict__ rec, int j, int self) {
+    const RecP2e q = rec[j];  // one dwordx3 load
+    RecView r;
+    r.el = q.el;
+    r.lo = q.lo;
+    r.hi = q.hi;
+    r.self = self;
+    return r;
+  }
+};
 template <int LROW>
 struct RecView<RecP2c, LROW> : RecP2cView<LROW> {
   __device__ __forceinline__ static RecView load(const RecP2c *__restrict__ rec, int j,
@@ -495,8 +525,8 @@ __device__ __forceinline__ void row_values(const T e[6], T md, T v[NL]) {
 }
 
 // ROWS rows per workgroup, LPR lanes per row, the block's CSR entries staged
-// in LDS (plus a 256-slot dummy region that takes the contributions to
-// eliminated columns).  The lanes of a row take its
+// in LDS (plus, with ELIM, a slot at index `dummy` that takes the
+// contributions to eliminated columns).  The lanes of a row take its
 // incidences in turn (lane sub takes sub, sub + LPR, ...) and add their rows
 // of the element matrices in LPR ordered phases, so every entry sums its
 // contributions in incidence order, independent of ROWS and LPR.  An
@@ -554,7 +584,10 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? 6 : LROW == 0 ? 3
   const int rr = kChunked ? min(r, r1 - 1) : r;
   // compact records leave out the row's own column: its position is the
   // row's diagonal
-  constexpr bool kSelf = std::is_same<Rec, RecP2c>::value || std::is_same<Rec, RecP1p>::value;
+  constexpr bool kSelf = std::is_same<Rec, RecP2c>::value || std::is_same<Rec, RecP1p>::value ||
+                        std::is_same<Rec, RecP2e>::value;
+  // records that carry their P2 patch words
+  constexpr bool kRecLv = std::is_same<Rec, RecP2e>::value;
   using RV = RecView<Rec, LROW>;
   int base = 0, iend = 0, i = 0, self = 0;
   if (kChunked) {
@@ -568,7 +601,27 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? 6 : LROW == 0 ? 3
   // to it: a row's records always point into its own block's patch)
   double *PX = nullptr;
   uint32_t lvr = 0;
-  int npm = 0;
+  int npm = 0, p0 = 0;
+  // coordinates of patch vertex lv (staged, or past the staged ones): the
+  // LDS read is unconditional (clamped index) and only the memory read sits
+  // in a branch, a form the compiler keeps apart
+  // (vertex-row launches stage every patch in full: their patches are alike)
+  constexpr bool kOvf = NL == 10 && LROW == 4;
+  auto patch_xyz = [&](int lv) {
+    const int ls = kOvf ? min(lv, pc.slots - 1) : lv;
+    double3 q;
+    q.x = PX[ls];
+    q.y = PX[pc.slots + ls];
+    q.z = PX[2 * pc.slots + ls];
+    if (kOvf && lv >= pc.slots) {
+      const size_t v = (size_t)pc.vid[p0 + lv];
+      const dbl2_a8 xy = *reinterpret_cast<const dbl2_a8 *>(es.xyz + 3 * v);
+      q.x = xy.x;
+      q.y = xy.y;
+      q.z = es.xyz[3 * v + 2];
+    }
+    return q;
+  };
   // Patched rows: the lane's first KP incidences (records, then their
   // coefficients) are requested before the patch is staged, so that their
   // latency overlaps the staging and the barrier; a block then waits for
@@ -585,19 +638,27 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? 6 : LROW == 0 ? 3
     for (int k = 0; k < KP; ++k) {
       const int j = i + k * LPR;
       Rp[k] = RV::load(rec, j < iend ? j : lastp, self);
-      if constexpr (NL == 10) lvw[k] = pc.inc_lv[j < iend ? j : lastp];
+      if constexpr (kRecLv)
+        lvw[k] = Rp[k].patch_word();
+      else if constexpr (NL == 10)
+        lvw[k] = pc.inc_lv[j < iend ? j : lastp];
     }
     PX = reinterpret_cast<double *>(lds_raw +
-                                    ((((size_t)dummy + 256) * sizeof(T) + 7) & ~(size_t)7));
+                                    ((((size_t)dummy + (ELIM ? 2 : 0)) * sizeof(T) + 7) & ~(size_t)7));
     const int pb = pc.pb0 + (r0 - row_begin) / kPatchRows;
-    const int p0 = pc.ptr[pb], np = pc.ptr[pb + 1] - p0;
+    p0 = pc.ptr[pb];
+    const int np = pc.ptr[pb + 1] - p0;
     npm = max(np - 1, 0);
+    // the first pc.slots vertices are staged; a P2 edge-row block with a
+    // larger patch (a rare outlier: the launch's LDS is sized for its
+    // typical blocks) reads the coordinates of the others from memory
+    const int ns_p = kOvf ? min(np, pc.slots) : np;
     lvr = pc.row_lv[rr];
     // up to three vertices per lane with their loads in flight together
     constexpr int KV = 3;
     int vid[KV];
 #pragma unroll
-    for (int u = 0; u < KV; ++u) vid[u] = lane + u * NT < np ? pc.vid[p0 + lane + u * NT] : 0;
+    for (int u = 0; u < KV; ++u) vid[u] = lane + u * NT < ns_p ? pc.vid[p0 + lane + u * NT] : 0;
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
       const int el = Rp[k].el >> 4;
@@ -613,8 +674,8 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? 6 : LROW == 0 ? 3
     };
 #pragma unroll
     for (int u = 0; u < KV; ++u)
-      if (lane + u * NT < np) put(lane + u * NT, vid[u]);
-    for (int p = lane + KV * NT; p < np; p += NT) put(p, pc.vid[p0 + p]);
+      if (lane + u * NT < ns_p) put(lane + u * NT, vid[u]);
+    for (int p = lane + KV * NT; p < ns_p; p += NT) put(p, pc.vid[p0 + p]);
   }
   if constexpr (GACC) {
     // each lane clears its own row (it is the only one to touch it)
@@ -631,9 +692,10 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? 6 : LROW == 0 ? 3
 #pragma unroll
     for (int h = 0; h < (LROW == 0 ? 1 : 2); ++h) {
       const int lv = min((int)((lvr >> (16 * h)) & 0xFFFFu), npm);
-      own[h][0] = PX[lv];
-      own[h][1] = PX[pc.slots + lv];
-      own[h][2] = PX[2 * pc.slots + lv];
+      const double3 q = patch_xyz(lv);
+      own[h][0] = q.x;
+      own[h][1] = q.y;
+      own[h][2] = q.z;
     }
   }
   if (r < r1) {
@@ -767,7 +829,10 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? 6 : LROW == 0 ? 3
 #pragma unroll
           for (int k = 0; k < KP; ++k) {
             Rp[k] = RV::load(rec, at(i + k * LPR), self);
-            if constexpr (NL == 10) lvw[k] = pc.inc_lv[at(i + k * LPR)];
+            if constexpr (kRecLv)
+              lvw[k] = Rp[k].patch_word();
+            else if constexpr (NL == 10)
+              lvw[k] = pc.inc_lv[at(i + k * LPR)];
           }
 #pragma unroll
           for (int k = 0; k < KP; ++k) {
@@ -806,9 +871,10 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? 6 : LROW == 0 ? 3
               else
                 lv = (int)((lvw[k] >> (10 * (m - M0))) & 0x3FFu);
               lv = min(lv, npm);
-              P[m][0] = PX[lv];
-              P[m][1] = PX[pc.slots + lv];
-              P[m][2] = PX[2 * pc.slots + lv];
+              const double3 q = patch_xyz(lv);
+              P[m][0] = q.x;
+              P[m][1] = q.y;
+              P[m][2] = q.z;
             }
             T f[8];
             edge_factors<T, MASS>(P, ksp[k], kmp[k], f);
@@ -922,6 +988,10 @@ template <int V>
 using IC = std::integral_constant<int, V>;
 template <bool V>
 using BC = std::integral_constant<bool, V>;
+template <typename R>
+struct RecTag {
+  using type = R;
+};
 
 // Whether an assembly of S needs the factor kernel: P1 rows and patched FP64
 // P2 rows form the factors in place.
@@ -975,26 +1045,32 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
           c->nT, c->tets, c->xyz, ksc, ksf, kmc, kmf, geo);
     MG_HIP(hipGetLastError());
   }
-  const Rec *rec = (const Rec *)S.inc_rec;
-  for (const RowSegment &sg : S.segs) {
+  // one segment's launch with the records read as R
+  auto seg_launch = [&](const RowSegment &sg, auto rec_tag) {
+    using R = typename decltype(rec_tag)::type;
+    const R *rec = (const R *)S.inc_rec;
     const int nblk = (sg.row_end - sg.row_begin + sg.rows - 1) / sg.rows;
-    // staged entries, the 256-slot region of eliminated columns, then (at
-    // 8-byte alignment) the vertex patch coordinates
+    // staged entries, two slots that take the contributions to eliminated
+    // columns (interior systems), then (at 8-byte alignment) the coordinates
+    // of the first sg.patch vertices of each block's patch
     const bool patched = use_patch(sg, NL, sizeof(T));
-    size_t lds = sg.gacc ? 0 : sizeof(T) * (size_t)(sg.lds_slots + 256);
+    size_t lds = sg.gacc ? 0 : sizeof(T) * (size_t)(sg.lds_slots + (elim ? 2 : 0));
     if (patched) lds = ((lds + 7) & ~(size_t)7) + 24 * (size_t)sg.patch;
     const PatchSrc pc{S.patch_ptr, S.patch_vid, S.row_lv, S.inc_lv, sg.patch, sg.pb0};
     auto go = [&](auto lr, auto ms, auto ex, auto rw, auto lp, auto ga, auto pt) {
+      // the grid and the LDS size come from the segment's block rows
+      MG_CHECK(decltype(rw)::value == sg.rows, MG_ERR_STATE,
+               "assembly: a launch shape differs from its segment's row blocks");
       if (elim)
         k_asm_rows<NL, decltype(lr)::value, T, decltype(ms)::value, decltype(ex)::value, true,
-                   decltype(rw)::value, decltype(lp)::value, Rec, decltype(ga)::value,
+                   decltype(rw)::value, decltype(lp)::value, R, decltype(ga)::value,
                    decltype(pt)::value>
             <<<nblk, decltype(rw)::value * decltype(lp)::value, lds, c->stream>>>(
                 sg.row_begin, sg.row_end, sg.lds_slots, ep, S.rowptr, S.inc_ptr, S.diagpos, rec, geo,
                 es, pc, vals);
       else
         k_asm_rows<NL, decltype(lr)::value, T, decltype(ms)::value, decltype(ex)::value, false,
-                   decltype(rw)::value, decltype(lp)::value, Rec, decltype(ga)::value,
+                   decltype(rw)::value, decltype(lp)::value, R, decltype(ga)::value,
                    decltype(pt)::value>
             <<<nblk, decltype(rw)::value * decltype(lp)::value, lds, c->stream>>>(
                 sg.row_begin, sg.row_end, sg.lds_slots, ep, S.rowptr, S.inc_ptr, S.diagpos, rec, geo,
@@ -1011,8 +1087,8 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
       }
       if constexpr (decltype(lr)::value == 0) {
         const int lpr = sizeof(T) == 4 ? sg.lpr32 : sg.lpr;
-        if constexpr (std::is_same<Rec, RecP1p>::value ||
-                      (std::is_same<Rec, RecP2c>::value && sizeof(T) == 8)) {
+        if constexpr (std::is_same<R, RecP1p>::value ||
+                      (std::is_same<R, RecP2c>::value && sizeof(T) == 8)) {
           if (patched) {
             const BC<true> pt{};
             if (lpr == 4)
@@ -1031,13 +1107,20 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
         else
           go(lr, ms, ex, IC<64>{}, IC<1>{}, st, st);
       } else {
-        if constexpr (std::is_same<Rec, RecP2c>::value && sizeof(T) == 8) {
+        if constexpr ((std::is_same<R, RecP2c>::value || std::is_same<R, RecP2e>::value) &&
+                      sizeof(T) == 8) {
           if (patched) {
             const BC<true> pt{};
             sg.lpr == 2 ? go(lr, ms, ex, IC<kPatchRows>{}, IC<2>{}, st, pt)
                         : go(lr, ms, ex, IC<kPatchRows>{}, IC<1>{}, st, pt);
             return;
           }
+        }
+        if constexpr (std::is_same<R, RecP2e>::value) {
+          // RecP2e rows lie in kPatchRows-row patch blocks
+          sg.lpr == 2 ? go(lr, ms, ex, IC<kPatchRows>{}, IC<2>{}, st, st)
+                      : go(lr, ms, ex, IC<kPatchRows>{}, IC<1>{}, st, st);
+          return;
         }
         if (sg.rows == 256)
           sg.lpr == 2 ? go(lr, ms, ex, IC<256>{}, IC<2>{}, st, st)
@@ -1056,11 +1139,25 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
       else
         mass ? shape(lr, BC<true>{}, BC<false>{}) : shape(lr, BC<false>{}, BC<false>{});
     };
-    if (sg.lrow == 0)
-      terms(IC<0>{});
-    else if constexpr (NL == 10)
-      terms(IC<4>{});
+    if constexpr (std::is_same<R, RecP2e>::value) {
+      terms(IC<4>{});  // edge-dof rows only
+    } else {
+      if (sg.lrow == 0)
+        terms(IC<0>{});
+      else if constexpr (NL == 10)
+        terms(IC<4>{});
+    }
     MG_HIP(hipGetLastError());
+  };
+  for (const RowSegment &sg : S.segs) {
+    // P2 edge-dof rows of a system with RecP2e edge records
+    if constexpr (std::is_same<Rec, RecP2c>::value) {
+      if (sg.lrow == 4 && S.edge5) {
+        seg_launch(sg, RecTag<RecP2e>{});
+        continue;
+      }
+    }
+    seg_launch(sg, RecTag<Rec>{});
   }
 }
 

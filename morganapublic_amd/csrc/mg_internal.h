@@ -54,6 +54,17 @@ struct alignas(4) RecP1p {
   int32_t el;
   uint32_t lo, hi;
 };
+// Patched P2 edge-row record (12 bytes, a RecP2c system whose edge-dof rows
+// all have <= 31 entries and whose edge-row patches have <= 511 vertices):
+// el as in RecP2c; (lo, hi) hold the 5-bit positions of the 9 columns other
+// than the row's own (bits 0-44, increasing relabelled basis, 31: eliminated
+// column), the 9-bit patch indices of relabelled vertices 2 and 3 (bits
+// 45-53, 54-62) and, in bit 63, whether relabelled vertex 0 is the row's
+// second endpoint.  It replaces a RecP2c record plus its 4-byte patch word.
+struct alignas(4) RecP2e {
+  int32_t el;
+  uint32_t lo, hi;
+};
 enum RecKind { kRecP1 = 1, kRecP1w, kRecP2, kRecP2c, kRecP2w, kRecP1p };
 
 // Vertex patches of row blocks: the sorted distinct vertices of the
@@ -126,7 +137,7 @@ struct System {
   int32_t *colidx = nullptr;   // nnz
   int32_t *diagpos = nullptr;  // nrows, CSR index of the diagonal entry
   int32_t *inc_ptr = nullptr;  // nrows + 1 (row -> incidences)
-  void *inc_rec = nullptr;     // n_inc records (RecP1, RecP2 or RecP2c), sorted by (row, l, e)
+  void *inc_rec = nullptr;     // n_inc records (RecP1, RecP1w, RecP1p, RecP2, RecP2c (+ RecP2e), RecP2w), sorted by (row, l, e)
   int rec_words = 0;           // int32 words per record: 2 (RecP1), 3 (RecP1w, RecP1p, RecP2c), 4 (RecP2), 6 (RecP2w)
   int rec_kind = 0;            // RecKind
   std::vector<RowSegment> segs;
@@ -143,6 +154,10 @@ struct System {
   int32_t *patch_vid = nullptr;
   uint32_t *row_lv = nullptr;
   uint32_t *inc_lv = nullptr;
+  // the records of the P2 edge-dof rows (incidences from edge_inc0 on) are
+  // in the RecP2e form, which carries their patch words
+  bool edge5 = false;
+  int64_t edge_inc0 = 0;
   // SpMV row blocks: each holds <= kSpmvNnz entries and <= kSpmvRows rows
   BlkDesc *sdesc = nullptr;
   int32_t n_sblk = 0;

@@ -607,6 +607,31 @@ __global__ __launch_bounds__(256) void k_patch(int64_t nvrows, int nvb, int64_t 
   }
 }
 
+// RecP2e edge-row records from the RecP2c records [i0, n_inc) and their patch
+// words: 7-bit positions re-packed in 5 bits (0x7F -> 31, eliminated), the
+// 10-bit patch indices of relabelled vertices 2 and 3 in 9 bits, and the
+// orientation bit.  bad counts fields that do not fit.
+__global__ void k_edge5(int64_t i0, int64_t n_inc, RecP2c *__restrict__ rec,
+                        const uint32_t *__restrict__ inc_lv, int *__restrict__ bad) {
+  const int64_t i = i0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_inc) return;
+  const RecP2c q = rec[i];
+  const uint64_t w7 = (uint64_t)q.lo | ((uint64_t)q.hi << 32);
+  uint64_t w = 0;
+  int nbad = 0;
+  for (int k = 0; k < 9; ++k) {
+    const uint32_t p = (uint32_t)(w7 >> (7 * k)) & 0x7Fu;
+    nbad += (p != 0x7Fu && p >= 31u);
+    w |= (uint64_t)(p == 0x7Fu ? 31u : (p & 31u)) << (5 * k);
+  }
+  const uint32_t lv = inc_lv[i];
+  const uint32_t a = lv & 0x3FFu, b = (lv >> 10) & 0x3FFu, sw = (lv >> 30) & 1u;
+  nbad += (a > 511u) + (b > 511u);
+  w |= ((uint64_t)(a & 511u) << 45) | ((uint64_t)(b & 511u) << 54) | ((uint64_t)sw << 63);
+  if (nbad) atomicAdd(bad, nbad);
+  reinterpret_cast<RecP2e *>(rec)[i] = RecP2e{q.el, (uint32_t)w, (uint32_t)(w >> 32)};
+}
+
 // Vertex patches of a RecP1p / RecP2c system (vertex rows [0, nvrows), P2
 // edge rows after them); returns the patch size of every patch block (0: no
 // patch).  A block gets one when its distinct vertices fit the index form
@@ -737,8 +762,24 @@ static void make_segments(System &S, const std::vector<int32_t> &rp, int64_t nvr
         if (x <= run) return;
         push(run, x, lrow, kPatchRows, lpr, lpr32, rk == 1);
         if (rk == 2) {
-          S.segs.back().patch = mp;
-          S.segs.back().pb0 = pb0 + (int)((run - a) / kPatchRows);
+          // a P2 edge-row launch stages the patches of all but the largest
+          // 0.5% of its blocks in full (the others read their remaining
+          // vertices from memory), so one outlier does not size every
+          // block's LDS; MGFE_PATCH_STAGE > 0: stage at most that many
+          // vertices.  Vertex-row launches stage every patch in full.
+          const int q0 = pb0 + (int)((run - a) / kPatchRows);
+          const int q1 = pb0 + (int)((x - a + kPatchRows - 1) / kPatchRows);
+          int slots = mp;
+          if (lrow == 4) {
+            std::vector<int32_t> u(psz->begin() + q0, psz->begin() + q1);
+            std::sort(u.begin(), u.end());
+            const size_t k = std::min(u.size() - 1, (size_t)(0.995 * (double)u.size()));
+            slots = std::max(1, u[k]);
+            const int cap = env_int("MGFE_PATCH_STAGE", 0);
+            if (cap > 0) slots = std::min(slots, cap);
+          }
+          S.segs.back().patch = slots;
+          S.segs.back().pb0 = q0;
         }
         run = x;
       };
@@ -926,6 +967,32 @@ void build_pattern(Ctx *c, int sys) {
                                          ? build_patches<RecP1p>(c, S, rp, nvrows)
                                          : build_patches<RecP2c>(c, S, rp, nvrows);
     make_segments(S, rp, nvrows, nl, &psz);
+    // P2 edge-dof rows of <= 31 entries whose patches have <= 511 vertices:
+    // their records take the RecP2e form, patch words included (12 bytes per
+    // incidence instead of 16; MGFE_EDGE5=0 keeps the separate patch words)
+    if (S.rec_kind == kRecP2c && S.inc_lv && nvrows < nrows && env_int("MGFE_EDGE5", 1) != 0) {
+      int max_edge = 0, max_patch = 0;
+      for (int64_t r = nvrows; r < nrows; ++r) max_edge = std::max(max_edge, rp[r + 1] - rp[r]);
+      const size_t nvb = (size_t)((nvrows + kPatchRows - 1) / kPatchRows);
+      for (size_t b = nvb; b < psz.size(); ++b) max_patch = std::max(max_patch, psz[b]);
+      if (max_edge <= 31 && max_patch <= 511) {
+        int32_t i0 = 0;
+        MG_HIP(hipMemcpyAsync(&i0, S.inc_ptr + nvrows, sizeof(int32_t), hipMemcpyDeviceToHost,
+                              c->stream));
+        MG_HIP(hipStreamSynchronize(c->stream));
+        int *bad5 = dalloc<int>(c, 1);
+        MG_HIP(hipMemsetAsync(bad5, 0, sizeof(int), c->stream));
+        if (S.n_inc > i0)
+          k_edge5<<<blocks_for(S.n_inc - i0, 256), 256, 0, c->stream>>>(
+              i0, S.n_inc, (RecP2c *)S.inc_rec, S.inc_lv, bad5);
+        MG_HIP(hipGetLastError());
+        const int nb5 = read_back(c, bad5);
+        dfree(c, bad5);
+        MG_CHECK(nb5 == 0, MG_ERR_STATE, "pattern: an edge-row record does not fit the 5-bit form");
+        S.edge5 = true;
+        S.edge_inc0 = i0;
+      }
+    }
   } else {
     make_segments(S, rp, nvrows, nl, nullptr);
   }

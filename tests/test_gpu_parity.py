@@ -220,6 +220,62 @@ def test_assembly_launch_shapes_bit_identical(ctx, monkeypatch, order, rs, prec,
         assert _row_scaled_diff(ctx, sysid, out["0"][0], out["1"][0]) <= 1e-14
 
 
+@pytest.mark.parametrize("terms,rs,rm", [("stiff", 11, 15), ("both", 11, 15), ("both", 1, 4)])
+@pytest.mark.parametrize("prec", [L.PREC_F64, L.PREC_F32])
+@pytest.mark.parametrize("sysid", [L.SYS_FULL, L.SYS_INTERIOR])
+def test_edge_record_forms_bit_identical(ctx, monkeypatch, terms, rs, rm, prec, sysid):
+    # P2 edge-dof rows read 12-byte records that carry their patch words
+    # (5-bit positions) or 12-byte records plus separate 4-byte patch words
+    # (7-bit positions): the same positions and patch indices, so the same
+    # bits, for every lanes-per-row shape; the oracle pins the default form
+    m = W.kuhn_cube(6, jitter=0.2, seed=23)
+    k = np.exp(W.stream_normal(0, W.STREAM_K, m.nT))
+    t = L.TERM_STIFF | (L.TERM_MASS if terms == "both" else 0)
+    out = {}
+    for e5 in ("1", "0"):
+        for lpr in ("2", "1"):
+            monkeypatch.setenv("MGFE_EDGE5", e5)
+            monkeypatch.setenv("MGFE_EDGE_LPR", lpr)
+            setup(ctx, m, 2)
+            ctx.field_set(L.FIELD_STIFF, k)
+            ctx.assemble(sysid, t, stiff_coeff=L.COEFF_P0, stiff_rule=rs, mass_rule=rm, prec=prec)
+            out[(e5, lpr)] = ctx.values(sysid, prec=prec)
+    ref = out[("1", "2")]
+    for key, v in out.items():
+        assert np.array_equal(ref, v), key
+    if prec == L.PREC_F64 and terms == "stiff":
+        rp, ci = ctx.pattern(sysid)
+        orp, oci, ova = O.assemble(2, m, O.OP_STIFF, c_stiff=k, rule_stiff=4)
+        if sysid == L.SYS_INTERIOR:
+            orp, oci, ova = O.restrict(orp, oci, ova, O.dirichlet_mask(2, m, list(ALL_FACES)))
+        assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+        assert np.abs(ref - ova).max() <= 1e-12 * np.abs(ova).max()
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("prec", [L.PREC_F64, L.PREC_F32])
+@pytest.mark.parametrize("sysid", [L.SYS_FULL, L.SYS_INTERIOR])
+def test_patch_overflow_bit_identical(ctx, monkeypatch, order, prec, sysid):
+    # a P2 edge-row launch stages at most a fixed number of each block's
+    # patch vertices in LDS and reads the coordinates of the others from
+    # memory: the same coordinates, so the same bits (vertex-row launches
+    # stage every patch in full and ignore the limit)
+    m = W.kuhn_cube(9 if order == 1 else 5, jitter=0.2, seed=29)
+    k = np.exp(W.stream_normal(0, W.STREAM_K, m.nT))
+    out = []
+    for cap in (None, "40", "1"):
+        if cap is None:
+            monkeypatch.delenv("MGFE_PATCH_STAGE", raising=False)
+        else:
+            monkeypatch.setenv("MGFE_PATCH_STAGE", cap)
+        setup(ctx, m, order)
+        ctx.field_set(L.FIELD_STIFF, k)
+        ctx.assemble(sysid, L.TERM_STIFF | L.TERM_MASS, stiff_coeff=L.COEFF_P0,
+                     stiff_rule=4 if order == 2 else 1, mass_rule=15, prec=prec)
+        out.append(ctx.values(sysid, prec=prec))
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+
+
 # record forms: (MGFE_PATCH, MGFE_PATCH_MAX): forms without patches;
 # patched records without patches; patches everywhere; patches on some row
 # blocks only
