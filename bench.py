@@ -116,30 +116,99 @@ def bytes_spmv(n, nnz, vb):
     return nnz * (vb + 4) + 4 * (n + 1) + 2 * vb * n
 
 
-def kernel_source_sha(name="mg_assemble.hip"):
-    import hashlib
-    with open(os.path.join(ROOT, "morganapublic_amd", "csrc", name), "rb") as f:
-        return hashlib.sha256(f.read()).hexdigest()[:16]
+TRAFFIC_PASSES = (
+    ("TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum"),
+    ("TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum"),
+)
+TRAFFIC_REPS = 5
 
 
-def measured_traffic(n):
-    """L2 <-> memory bytes of one FP64 assembly at this size from the committed
-    counter summary (rocprofv3 TCC_EA0_RDREQ_{32B,64B,128B} and
-    TCC_EA0_WRREQ(_64B), summed over the assembly's kernels), or None when
-    no summary exists or it was taken from another version of the assembly
-    kernels (its source hash differs from the file built here)."""
-    path = os.path.join(ROOT, "profiles", "assembly_p2_%d_traffic.json" % n)
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        d = json.load(f)
-    if d.get("source_sha") != kernel_source_sha():
-        return None
-    return d.get("bytes_per_assembly")
+def traffic_child(n):
+    """Run under rocprofv3 --pmc by measure_traffic: the FP64 assembly of the
+    full n^3 P2 system, TRAFFIC_REPS times after a short settle."""
+    p = W.config_problem(2, n=n)
+    m = p.mesh
+    ctx = L.Context(0)
+    ctx.mesh_set(m.xyz, m.tets, m.btri, m.bgeo)
+    ctx.dofmap_build(2)
+    ctx.pattern_build(L.SYS_FULL)
+    ctx.field_set(L.FIELD_STIFF, p.k)
+    for _ in range(TRAFFIC_REPS):
+        ctx.assemble(L.SYS_FULL, L.TERM_STIFF, stiff_coeff=L.COEFF_P0, stiff_rule=11)
+    ctx.synchronize()
+    ctx.close()
 
 
-def bench_config5(ctx, timer, keep, n=128):
-    """128^3 P1, lognormal k: forward + adjoint solve with both FP32 + FP64
+def measure_traffic(n, timeout=150):
+    """Bytes between L2 and memory of one FP64 assembly, measured in this run:
+    two rocprofv3 counter passes over a child process (traffic_child), one per
+    counter group (request counts by size: read = 128 RDREQ_128B + 64
+    RDREQ_64B + 32 RDREQ_32B, write = 64 WRREQ_64B + 32 (WRREQ - WRREQ_64B)),
+    summed over the assembly kernels (median per kernel over the dispatches).
+    None (with the reason) when the profiler is unavailable or fails."""
+    import csv
+    import shutil
+    import statistics
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None, "rocprofv3 not found"
+    env = dict(os.environ, TMPDIR="/tmp")
+    per = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        for k, counters in enumerate(TRAFFIC_PASSES):
+            out = os.path.join(td, "p%d" % k)
+            cmd = [prof, "--pmc", *counters, "--output-format", "csv", "-d", out, "-o", "pmc",
+                   "--", sys.executable, os.path.abspath(__file__), "--traffic-child",
+                   "--n", str(n)]
+            # own process group, so that a pass past its time limit is ended
+            # together with the program it profiles
+            pr = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL,
+                                  stderr=subprocess.DEVNULL, start_new_session=True)
+            try:
+                rc = pr.wait(timeout=timeout)
+            except subprocess.TimeoutExpired:
+                os.killpg(pr.pid, 9)
+                pr.wait()
+                return None, "rocprofv3 pass %d timed out" % k
+            files = []
+            for root, _, names in os.walk(out):
+                files += [os.path.join(root, f) for f in names if f.endswith("counter_collection.csv")]
+            if not files:
+                return None, "rocprofv3 pass %d: no counter file (rc %d)" % (k, rc)
+            for f in files:
+                with open(f) as fh:
+                    for row in csv.DictReader(fh):
+                        name = row["Kernel_Name"]
+                        if "k_asm_rows" not in name and "k_elem_factors" not in name:
+                            continue
+                        key = name[:name.find("(")] if "(" in name else name
+                        per.setdefault(key, {}).setdefault(row["Counter_Name"], []).append(
+                            float(row["Counter_Value"]))
+    if not per:
+        return None, "no assembly kernel in the counter files"
+    kernels, rd_tot, wr_tot = [], 0.0, 0.0
+    for key, cs in per.items():
+        med = {c: statistics.median(v) for c, v in cs.items()}
+        calls = max(len(v) for v in cs.values())
+        rd = (128 * med.get("TCC_EA0_RDREQ_128B_sum", 0) + 64 * med.get("TCC_EA0_RDREQ_64B_sum", 0)
+              + 32 * med.get("TCC_EA0_RDREQ_32B_sum", 0))
+        wr64 = med.get("TCC_EA0_WRREQ_64B_sum", 0)
+        wr = 64 * wr64 + 32 * (med.get("TCC_EA0_WRREQ_sum", 0) - wr64)
+        # launches per assembly: dispatches of this kernel over the repetitions
+        per_asm = calls / TRAFFIC_REPS
+        rd_tot += rd * per_asm
+        wr_tot += wr * per_asm
+        kernels.append({"kernel": key[:160], "launches_per_assembly": per_asm,
+                        "read_bytes": int(rd), "write_bytes": int(wr)})
+    return {"bytes_per_assembly": int(rd_tot + wr_tot), "read_bytes": int(rd_tot),
+            "write_bytes": int(wr_tot), "kernels": kernels}, None
+
+
+def bench_config5(ctx, timer, keep, n=128, solve=True):
+    """128^3 P1, lognormal k: the assembly alone (FP64 and FP32, interior
+    system), then (solve) forward + adjoint solve with both FP32 + FP64
     refinement methods to 1e-10, vs the FP64 plain method; dJ/dk for
     J = ||u - u_obs||^2."""
     p = W.config_problem(5, n=n)
@@ -176,6 +245,10 @@ def bench_config5(ctx, timer, keep, n=128):
             rp5, ci5 = ctx.pattern(L.SYS_INTERIOR)
             keep["p1_128"] = (p, rp5, ci5, ctx.values(L.SYS_INTERIOR), ctx.dirichlet_mask())
         log("cfg5 P1 assembly %s: %.1f us (%.0f M elem/s)" % (name, 1e3 * ms, m.nT / ms / 1e3))
+    if not solve:
+        for ptr in (db, du, dl, dg):
+            timer.free(ptr)
+        return out
     grads = {}
     for name, mode in (("f64", L.SOLVE_F64), ("f32_refine", L.SOLVE_F32_REFINE),
                        ("f32_reliable", L.SOLVE_F32_RELIABLE)):
@@ -380,9 +453,19 @@ def main():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter passes that measure the assembly's memory traffic")
+    ap.add_argument("--assembly-only", action="store_true",
+                    help="the 64^3 P2 and 128^3 P1 assemblies only (no solves, extras or CPU baseline)")
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the 96^3 drift-diffusion and 128^3 gradient extras")
     args = ap.parse_args()
+    if args.traffic_child:
+        traffic_child(args.n)
+        return
+    if args.assembly_only:
+        args.no_solve = args.no_cpu = True
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -453,7 +536,6 @@ def main():
     log("device copy 1 GiB: %.3f ms, %.0f GB/s (read + write)" % (ms_copy, copy_gbs))
     timer.free(src)
     timer.free(dst)
-    traffic = measured_traffic(args.n)
 
     # FP32 assembly (inner-solve matrix)
     assemble(prec=L.PREC_F32)
@@ -581,6 +663,16 @@ def main():
     if not args.no_solve and not args.no_configs:
         out_extra["config5_gradient_128"] = bench_config5(ctx, timer, keep if not args.no_cpu else {})
         out_extra["config4_drift_diffusion_96"] = bench_config4(ctx, timer)
+    elif args.assembly_only and not args.no_configs:
+        out_extra["config5_gradient_128"] = bench_config5(ctx, timer, {}, solve=False)
+    traffic, traffic_note = None, "not measured (--no-traffic)"
+    if not args.no_traffic:
+        traffic, traffic_note = measure_traffic(args.n)
+        if traffic:
+            log("assembly traffic (L2 <-> memory, counted in this run): %.1f MB read + %.1f MB "
+                "written per FP64 assembly" % (traffic["read_bytes"] / 1e6, traffic["write_bytes"] / 1e6))
+        else:
+            log("assembly traffic not measured: %s" % traffic_note)
     cpu = None
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -608,12 +700,13 @@ def main():
                    "dofs_interior": nint, "nnz_interior": nnz_int},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic,
+                     "traffic": traffic["bytes_per_assembly"] if traffic else None,
                      "kernel": "one FP64 assembly = k_asm_rows over the vertex-dof rows + "
                                "k_asm_rows over the edge-dof rows (element factors formed in "
                                "place from each row block's vertex patch in LDS); algorithmic "
                                "bytes %d" % b_asm,
                      "copy_GBps": round(copy_gbs, 1)},
+        "assembly_traffic": traffic if traffic else {"note": traffic_note},
         "settle_steps": settle,
         "assembly_f32_ms": round(ms_asm32, 4),
         "symbolic_s": round(t_sym, 3),

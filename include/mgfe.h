@@ -177,9 +177,11 @@ int mg_assemble(mg_ctx *ctx, int sys, const mg_operator *op, int prec);
  * kernels and the same values, bit for bit; the per-element factor records
  * go to a static 128 MiB device array (one per GPU, held until the process
  * ends) instead of context memory, and nothing is allocated or freed.
- * MG_ERR_SIZE when the records need more than the array: P2 takes 48 bytes
- * per element in FP64 (64 with a mass term) and half that in FP32, up to
- * 2.79 M (2.09 M) elements; FP64 P1 takes none.  The pattern must have been
+ * MG_ERR_SIZE when the records need more than the array.  Records are
+ * needed by FP32 P2 assemblies (24 bytes per element, 32 with a mass term:
+ * up to 5.59 M / 4.19 M elements) and by FP64 P2 assemblies of a pattern
+ * with row blocks that have no vertex patch (48 / 64 bytes: 2.79 M /
+ * 2.09 M); P1 and fully patched FP64 P2 assemblies take none.  The pattern must have been
  * built by mg_pattern_build (which allocates).  Every context on a GPU
  * shares that GPU's array: calls that overlap on one GPU (other streams or
  * host threads) race.                                                      */
