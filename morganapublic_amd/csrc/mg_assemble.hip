@@ -160,30 +160,36 @@ constexpr int geo_stride() {
 template <typename T, bool MASS>
 __device__ __forceinline__ void edge_factors(const double P[4][3], double ks, double km,
                                              T out[8]) {
+  // explicit fma only: every kernel that forms factors (the factor kernel,
+  // P1 rows, patched P2 rows, each launch shape) must round them alike,
+  // whatever the compiler would fuse in its context
+#pragma clang fp contract(off)
   double d[3][3];
 #pragma unroll
   for (int k = 0; k < 3; ++k)
 #pragma unroll
     for (int r = 0; r < 3; ++r) d[k][r] = P[k + 1][r] - P[0][r];
+  // a b - c e with one rounding of the product c e
+  auto x2 = [](double a, double b, double c, double e) { return fma(a, b, -(c * e)); };
   double n[4][3];
-  n[1][0] = d[1][1] * d[2][2] - d[1][2] * d[2][1];
-  n[1][1] = d[1][2] * d[2][0] - d[1][0] * d[2][2];
-  n[1][2] = d[1][0] * d[2][1] - d[1][1] * d[2][0];
-  n[2][0] = d[2][1] * d[0][2] - d[2][2] * d[0][1];
-  n[2][1] = d[2][2] * d[0][0] - d[2][0] * d[0][2];
-  n[2][2] = d[2][0] * d[0][1] - d[2][1] * d[0][0];
-  n[3][0] = d[0][1] * d[1][2] - d[0][2] * d[1][1];
-  n[3][1] = d[0][2] * d[1][0] - d[0][0] * d[1][2];
-  n[3][2] = d[0][0] * d[1][1] - d[0][1] * d[1][0];
+  n[1][0] = x2(d[1][1], d[2][2], d[1][2], d[2][1]);
+  n[1][1] = x2(d[1][2], d[2][0], d[1][0], d[2][2]);
+  n[1][2] = x2(d[1][0], d[2][1], d[1][1], d[2][0]);
+  n[2][0] = x2(d[2][1], d[0][2], d[2][2], d[0][1]);
+  n[2][1] = x2(d[2][2], d[0][0], d[2][0], d[0][2]);
+  n[2][2] = x2(d[2][0], d[0][1], d[2][1], d[0][0]);
+  n[3][0] = x2(d[0][1], d[1][2], d[0][2], d[1][1]);
+  n[3][1] = x2(d[0][2], d[1][0], d[0][0], d[1][2]);
+  n[3][2] = x2(d[0][0], d[1][1], d[0][1], d[1][0]);
 #pragma unroll
   for (int r = 0; r < 3; ++r) n[0][r] = -(n[1][r] + n[2][r] + n[3][r]);
-  const double det = d[0][0] * n[1][0] + d[0][1] * n[1][1] + d[0][2] * n[1][2];
+  const double det = fma(d[0][0], n[1][0], fma(d[0][1], n[1][1], d[0][2] * n[1][2]));
   const double sc = ks / (6.0 * det);
   constexpr int EA[6] = {0, 1, 2, 0, 1, 2}, EB[6] = {1, 2, 0, 3, 3, 3};  // kEdgeV
 #pragma unroll
   for (int j = 0; j < 6; ++j)
-    out[j] = (T)(sc * (n[EA[j]][0] * n[EB[j]][0] + n[EA[j]][1] * n[EB[j]][1] +
-                       n[EA[j]][2] * n[EB[j]][2]));
+    out[j] = (T)(sc * fma(n[EA[j]][0], n[EB[j]][0],
+                          fma(n[EA[j]][1], n[EB[j]][1], n[EA[j]][2] * n[EB[j]][2])));
   out[6] = MASS ? (T)(km * det) : (T)0;
   out[7] = (T)0;
 }
@@ -614,11 +620,15 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? 6 : LROW == 0 ? 3
     q.y = PX[pc.slots + ls];
     q.z = PX[2 * pc.slots + ls];
     if (kOvf && lv >= pc.slots) {
+      // two 16-byte loads, (x, y) and (y, z): unlike the scalar LDS reads,
+      // so the compiler cannot merge the two branches into a flat load
+      // (which would wait for every load in flight)
       const size_t v = (size_t)pc.vid[p0 + lv];
       const dbl2_a8 xy = *reinterpret_cast<const dbl2_a8 *>(es.xyz + 3 * v);
+      const dbl2_a8 yz = *reinterpret_cast<const dbl2_a8 *>(es.xyz + 3 * v + 1);
       q.x = xy.x;
       q.y = xy.y;
-      q.z = es.xyz[3 * v + 2];
+      q.z = yz.y;
     }
     return q;
   };
