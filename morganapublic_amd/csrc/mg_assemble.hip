@@ -549,10 +549,11 @@ __device__ __forceinline__ void row_values(const T e[6], T md, T v[NL]) {
 // incidence order, so the result is the staged form's bit for bit.
 template <int NL, int LROW, typename T, bool MASS, bool EXACT, bool ELIM, int ROWS, int LPR,
           typename Rec, bool GACC = false, bool PATCH = false>
-// patched P1 rows: at most 80 VGPRs, so 6 waves per SIMD fit, as many as
-// the LDS of their 64-row blocks allows (P2 blocks stage more entries, and
-// their LDS allows about 3 waves per SIMD for vertex rows, 4 for edge rows)
-__global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? 6 : LROW == 0 ? 3 : 4) void k_asm_rows(
+// patched P1 rows: at most 80 VGPRs (FP64), so 6 waves per SIMD fit (5:
+// +1.4%, 7: +20% from spills at 128^3); FP32 P1 rows at 5 waves per SIMD
+// (-5% against 6).  P2 blocks stage more entries, and their LDS allows about
+// 3 waves per SIMD for vertex rows, 4 for edge rows
+__global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ? 5 : 6) : LROW == 0 ? 3 : 4) void k_asm_rows(
     int32_t row_begin, int32_t row_end, int32_t dummy, EdgePack ep,
     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ inc_ptr,
     const int32_t *__restrict__ diagpos, const Rec *__restrict__ rec, const T *__restrict__ geo,
