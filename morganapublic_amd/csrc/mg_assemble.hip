@@ -254,6 +254,21 @@ struct PatchSrc {
   const uint32_t *inc_lv;
   int slots;
   int pb0;  // patch block of the segment's first row
+  // the ELL copy of the records (ell != 0: laid out for this launch's lanes
+  // per row): a lane's step-k record of the segment's block q at
+  // ell_rec[ell_base + (q * ell_steps + k) * NT + lane]
+  const void *ell_rec;
+  const uint32_t *ell_lv;
+  int64_t ell_base;
+  int ell_steps;
+  int ell;
+  // padded patch vertex ids (vid_pad non-null): the segment's block q stages
+  // vid_pad[vpad_base + q * vpad_stride + p], p < slots
+  const int32_t *vid_pad;
+  int64_t vpad_base;
+  int vpad_stride;
+  // RecP1q records: the lowest element of each patch block
+  const int32_t *ebase;
 };
 
 struct ElemSrc {
@@ -379,6 +394,30 @@ struct RecView<RecP1p, LROW> {
     r.el = q.el;
     r.lo = q.lo;
     r.hi = q.hi;
+    r.self = self;
+    return r;
+  }
+};
+// 8-byte patched P1 (ELL copies only): el = the element's offset from its
+// block's lowest, times 16; own column (basis 0) = the diagonal, `self`
+template <int LROW>
+struct RecView<RecP1q, LROW> {
+  static constexpr int kElim = 31;
+  int32_t el;
+  uint32_t lo, hi;
+  int self;
+  __device__ __forceinline__ int pos(int m) const {
+    return m == 0 ? self : (int)((lo >> (5 * (m - 1))) & 31u);
+  }
+  __device__ __forceinline__ int lv(int o) const {
+    return (int)(o == 0 ? (lo >> 15) & 511u : o == 1 ? ((lo >> 24) | (hi << 8)) & 511u : (hi >> 1) & 511u);
+  }
+  __device__ __forceinline__ static RecView load(const RecP1q *__restrict__ rec, int j, int self) {
+    const uint2 q = reinterpret_cast<const uint2 *>(rec)[j];  // one dwordx2 load
+    RecView r;
+    r.el = (int32_t)(q.y >> 10) << 4;
+    r.lo = q.x;
+    r.hi = q.y;
     r.self = self;
     return r;
   }
@@ -592,7 +631,13 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
   // compact records leave out the row's own column: its position is the
   // row's diagonal
   constexpr bool kSelf = std::is_same<Rec, RecP2c>::value || std::is_same<Rec, RecP1p>::value ||
-                        std::is_same<Rec, RecP2e>::value;
+                        std::is_same<Rec, RecP2e>::value || std::is_same<Rec, RecP1q>::value;
+  // RecP1q records exist only as ELL copies, their elements relative to the
+  // block's lowest (eb0)
+  constexpr bool kQ = std::is_same<Rec, RecP1q>::value;
+  static_assert(!kQ || PATCH, "RecP1q records: patched blocks");
+  const bool ell = kQ || pc.ell;
+  int eb0 = 0;
   // records that carry their P2 patch words
   constexpr bool kRecLv = std::is_same<Rec, RecP2e>::value;
   using RV = RecView<Rec, LROW>;
@@ -643,20 +688,38 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
   uint32_t lvw[KP];  // P2 patch words
   double ksp[KP], kmp[KP];
   int lastp = 0;
+  // ELL records: the block's first slot and last step
+  int64_t eb = 0;
+  int emax = 0;
+  // record source of the lane's incidence step `st` (CSR position j when
+  // the records are read in CSR order)
+  auto rec_src = [&](int st, int j) -> int64_t {
+    return ell ? eb + (int64_t)min(st, emax) * NT + lane : (int64_t)j;
+  };
   if constexpr (PATCH) {
     lastp = iend > i - sub ? iend - 1 : 0;
+    const int pb = pc.pb0 + (r0 - row_begin) / kPatchRows;
+    const Rec *rsrc = rec;
+    const uint32_t *lsrc = pc.inc_lv;
+    if (ell) {
+      eb = pc.ell_base + (int64_t)((r0 - row_begin) / kPatchRows) * pc.ell_steps * NT;
+      emax = pc.ell_steps - 1;
+      rsrc = reinterpret_cast<const Rec *>(pc.ell_rec);
+      lsrc = pc.ell_lv;
+    }
+    if constexpr (kQ) eb0 = pc.ebase[pb];
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
       const int j = i + k * LPR;
-      Rp[k] = RV::load(rec, j < iend ? j : lastp, self);
+      const int64_t src = rec_src(k, j < iend ? j : lastp);
+      Rp[k] = RV::load(rsrc, src, self);
       if constexpr (kRecLv)
         lvw[k] = Rp[k].patch_word();
       else if constexpr (NL == 10)
-        lvw[k] = pc.inc_lv[j < iend ? j : lastp];
+        lvw[k] = lsrc[src];
     }
     PX = reinterpret_cast<double *>(lds_raw +
                                     ((((size_t)dummy + (ELIM ? 2 : 0)) * sizeof(T) + 7) & ~(size_t)7));
-    const int pb = pc.pb0 + (r0 - row_begin) / kPatchRows;
     p0 = pc.ptr[pb];
     const int np = pc.ptr[pb + 1] - p0;
     npm = max(np - 1, 0);
@@ -665,14 +728,22 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
     // typical blocks) reads the coordinates of the others from memory
     const int ns_p = kOvf ? min(np, pc.slots) : np;
     lvr = pc.row_lv[rr];
-    // up to three vertices per lane with their loads in flight together
+    // up to three vertices per lane with their loads in flight together;
+    // from the padded ids (all pc.slots of them, whatever the block's patch
+    // size, so the loads do not wait for it) or the block's own
     constexpr int KV = 3;
     int vid[KV];
+    const int *vsrc = pc.vid + p0;
+    int nst = ns_p;
+    if (pc.vid_pad) {
+      vsrc = pc.vid_pad + pc.vpad_base + (int64_t)((r0 - row_begin) / kPatchRows) * pc.vpad_stride;
+      nst = pc.slots;
+    }
 #pragma unroll
-    for (int u = 0; u < KV; ++u) vid[u] = lane + u * NT < ns_p ? pc.vid[p0 + lane + u * NT] : 0;
+    for (int u = 0; u < KV; ++u) vid[u] = lane + u * NT < nst ? vsrc[lane + u * NT] : 0;
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
-      const int el = Rp[k].el >> 4;
+      const int el = (Rp[k].el >> 4) + eb0;
       ksp[k] = es.ksf ? es.ksf[el] : es.ksc;
       kmp[k] = MASS ? (es.kmf ? es.kmf[el] : es.kmc) : 0.0;
     }
@@ -685,8 +756,8 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
     };
 #pragma unroll
     for (int u = 0; u < KV; ++u)
-      if (lane + u * NT < ns_p) put(lane + u * NT, vid[u]);
-    for (int p = lane + KV * NT; p < ns_p; p += NT) put(p, pc.vid[p0 + p]);
+      if (lane + u * NT < nst) put(lane + u * NT, vid[u]);
+    for (int p = lane + KV * NT; p < nst; p += NT) put(p, vsrc[p]);
   }
   if constexpr (GACC) {
     // each lane clears its own row (it is the only one to touch it)
@@ -835,19 +906,22 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
       // the first KP incidences are in Rp/ksp/kmp already; each incidence's
       // relabelled vertices 1-3 come from the patch, vertex 0 is the row's
       bool first = true;
-      for (; i < iend; i += KP * LPR) {
+      const Rec *rsrc = ell ? reinterpret_cast<const Rec *>(pc.ell_rec) : rec;
+      const uint32_t *lsrc = ell ? pc.ell_lv : pc.inc_lv;
+      for (int c0 = 0; i < iend; i += KP * LPR, c0 += KP) {
         if (!first) {
 #pragma unroll
           for (int k = 0; k < KP; ++k) {
-            Rp[k] = RV::load(rec, at(i + k * LPR), self);
+            const int64_t src = rec_src(c0 + k, at(i + k * LPR));
+            Rp[k] = RV::load(rsrc, src, self);
             if constexpr (kRecLv)
               lvw[k] = Rp[k].patch_word();
             else if constexpr (NL == 10)
-              lvw[k] = pc.inc_lv[at(i + k * LPR)];
+              lvw[k] = lsrc[src];
           }
 #pragma unroll
           for (int k = 0; k < KP; ++k) {
-            const int el = Rp[k].el >> 4;
+            const int el = (Rp[k].el >> 4) + eb0;
             ksp[k] = es.ksf ? es.ksf[el] : es.ksc;
             kmp[k] = MASS ? (es.kmf ? es.kmf[el] : es.kmc) : 0.0;
           }
@@ -1059,7 +1133,8 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
   // one segment's launch with the records read as R
   auto seg_launch = [&](const RowSegment &sg, auto rec_tag) {
     using R = typename decltype(rec_tag)::type;
-    const R *rec = (const R *)S.inc_rec;
+    // (RecP1q records are read from the ELL copy alone)
+    const R *rec = std::is_same<R, RecP1q>::value ? nullptr : (const R *)S.inc_rec;
     const int nblk = (sg.row_end - sg.row_begin + sg.rows - 1) / sg.rows;
     // staged entries, two slots that take the contributions to eliminated
     // columns (interior systems), then (at 8-byte alignment) the coordinates
@@ -1067,11 +1142,27 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
     const bool patched = use_patch(sg, NL, sizeof(T));
     size_t lds = sg.gacc ? 0 : sizeof(T) * (size_t)(sg.lds_slots + (elim ? 2 : 0));
     if (patched) lds = ((lds + 7) & ~(size_t)7) + 24 * (size_t)sg.patch;
-    const PatchSrc pc{S.patch_ptr, S.patch_vid, S.row_lv, S.inc_lv, sg.patch, sg.pb0};
+    // the segment's first block in its ELL region
+    const int g = sg.lrow == 0 ? 0 : 1;
+    const int nvb = (int)((S.nvrows + kPatchRows - 1) / kPatchRows);
+    const int64_t q0 = g == 0 ? sg.pb0 : sg.pb0 - nvb;
+    PatchSrc pc{S.patch_ptr, S.patch_vid, S.row_lv, S.inc_lv, sg.patch, sg.pb0, S.ell_rec,
+                S.ell_lv,
+                S.ell_base[g] + q0 * S.ell_steps[g] * kPatchRows * std::max(1, S.ell_lpr[g]),
+                S.ell_steps[g], 0,
+                S.vid_pad && sg.patch <= S.vpad_stride[g] ? S.vid_pad : nullptr,
+                S.vpad_base[g] + q0 * S.vpad_stride[g], S.vpad_stride[g], S.ell_ebase};
     auto go = [&](auto lr, auto ms, auto ex, auto rw, auto lp, auto ga, auto pt) {
       // the grid and the LDS size come from the segment's block rows
       MG_CHECK(decltype(rw)::value == sg.rows, MG_ERR_STATE,
                "assembly: a launch shape differs from its segment's row blocks");
+      // patched launches read the ELL copy of the records when it is laid
+      // out for their lanes per row and holds their record form
+      pc.ell = decltype(pt)::value && S.ell_rec &&
+               S.ell_lpr[decltype(lr)::value == 0 ? 0 : 1] == decltype(lp)::value &&
+               S.ell_q == std::is_same<R, RecP1q>::value;
+      MG_CHECK((pc.ell || !std::is_same<R, RecP1q>::value), MG_ERR_STATE,
+               "assembly: 8-byte P1 records without their ELL copy");
       if (elim)
         k_asm_rows<NL, decltype(lr)::value, T, decltype(ms)::value, decltype(ex)::value, true,
                    decltype(rw)::value, decltype(lp)::value, R, decltype(ga)::value,
@@ -1145,10 +1236,17 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
       }
     };
     auto terms = [&](auto lr) {
+      auto sh = [&](auto ms, auto ex) {
+        // RecP1q: patched P1 blocks at 4 lanes per row (checked by the caller)
+        if constexpr (std::is_same<R, RecP1q>::value)
+          go(lr, ms, ex, IC<kPatchRows>{}, IC<4>{}, BC<false>{}, BC<true>{});
+        else
+          shape(lr, ms, ex);
+      };
       if (exact)
-        mass ? shape(lr, BC<true>{}, BC<true>{}) : shape(lr, BC<false>{}, BC<true>{});
+        mass ? sh(BC<true>{}, BC<true>{}) : sh(BC<false>{}, BC<true>{});
       else
-        mass ? shape(lr, BC<true>{}, BC<false>{}) : shape(lr, BC<false>{}, BC<false>{});
+        mass ? sh(BC<true>{}, BC<false>{}) : sh(BC<false>{}, BC<false>{});
     };
     if constexpr (std::is_same<R, RecP2e>::value) {
       terms(IC<4>{});  // edge-dof rows only
@@ -1161,6 +1259,15 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
     MG_HIP(hipGetLastError());
   };
   for (const RowSegment &sg : S.segs) {
+    // patched P1 blocks at 4 lanes per row read the 8-byte records of the
+    // ELL copy when it has them
+    if constexpr (std::is_same<Rec, RecP1p>::value) {
+      if (S.ell_q && !sg.gacc && use_patch(sg, NL, sizeof(T)) && S.ell_lpr[0] == 4 &&
+          (sizeof(T) == 4 ? sg.lpr32 : sg.lpr) == 4) {
+        seg_launch(sg, RecTag<RecP1q>{});
+        continue;
+      }
+    }
     // P2 edge-dof rows of a system with RecP2e edge records
     if constexpr (std::is_same<Rec, RecP2c>::value) {
       if (sg.lrow == 4 && S.edge5) {

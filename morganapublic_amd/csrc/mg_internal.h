@@ -65,6 +65,15 @@ struct alignas(4) RecP2e {
   int32_t el;
   uint32_t lo, hi;
 };
+// Patched P1 record of the ELL copy (8 bytes; systems whose rows have <= 31
+// entries, whose patches have <= 512 vertices and whose patch blocks each
+// reach elements within 2^22 of the block's lowest): bits 0-14 the 5-bit
+// positions of relabelled bases 1-3 (31: eliminated column), bits 15-41 the
+// 9-bit patch indices of relabelled vertices 1-3, bits 42-63 the element
+// minus the block's lowest element (System::ell_ebase).
+struct alignas(8) RecP1q {
+  uint32_t lo, hi;
+};
 enum RecKind { kRecP1 = 1, kRecP1w, kRecP2, kRecP2c, kRecP2w, kRecP1p };
 
 // Vertex patches of row blocks: the sorted distinct vertices of the
@@ -132,6 +141,7 @@ struct OpKey {
 struct System {
   bool built = false;
   int64_t nrows = 0, nnz = 0, n_inc = 0;
+  int64_t nvrows = 0;  // vertex-dof rows (rows [0, nvrows)); P1: all rows
   int max_row_nnz = 0;
   int32_t *rowptr = nullptr;   // nrows + 1
   int32_t *colidx = nullptr;   // nnz
@@ -158,6 +168,31 @@ struct System {
   // in the RecP2e form, which carries their patch words
   bool edge5 = false;
   int64_t edge_inc0 = 0;
+  // ELL copy of the patch blocks' incidence records, one region for the
+  // vertex-dof blocks (0) and one for the P2 edge-dof blocks (1), each with
+  // ell_steps[g] steps per block: slot (step k, lane t) of the region's block
+  // q is ell_rec[ell_base[g] + (q * ell_steps[g] + k) * kPatchRows * lpr + t],
+  // lane t = (row in block) * lpr + sub holding the row's incidence
+  // sub + k * lpr (past a row's last incidence: a copy of its last record; a
+  // row without any: record 0), so that a block's records are addressed
+  // without loading anything first; ell_lv: the P2 patch words in the same
+  // order; ell_lpr[g]: the lanes per row it was laid out for (0: none)
+  void *ell_rec = nullptr;
+  uint32_t *ell_lv = nullptr;
+  int64_t ell_base[2] = {0, 0};
+  int ell_steps[2] = {0, 0};
+  int ell_lpr[2] = {0, 0};
+  // P1 systems: the ELL copy holds RecP1q records (8 bytes) instead of
+  // RecP1p, with each patch block's lowest element in ell_ebase[b]
+  bool ell_q = false;
+  int32_t *ell_ebase = nullptr;
+  // the first vpad_stride[g] patch vertex ids of region g's block q at
+  // vid_pad[vpad_base[g] + q * vpad_stride[g] + p] (past a block's patch: its
+  // first vertex), so that a block's patch is addressed without loading its
+  // extent first; built together with the ELL copy
+  int32_t *vid_pad = nullptr;
+  int64_t vpad_base[2] = {0, 0};
+  int vpad_stride[2] = {0, 0};
   // SpMV row blocks: each holds <= kSpmvNnz entries and <= kSpmvRows rows
   BlkDesc *sdesc = nullptr;
   int32_t n_sblk = 0;

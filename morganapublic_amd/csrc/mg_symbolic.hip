@@ -632,6 +632,83 @@ __global__ void k_edge5(int64_t i0, int64_t n_inc, RecP2c *__restrict__ rec,
   reinterpret_cast<RecP2e *>(rec)[i] = RecP2e{q.el, (uint32_t)w, (uint32_t)(w >> 32)};
 }
 
+// RecP1q form of a patched P1 record, its element taken relative to eb (the
+// lowest element of its block); nbad counts the fields that do not fit
+__device__ __forceinline__ RecP1q p1q_pack(const RecP1p &q, int eb, int &nbad) {
+  uint64_t w = 0;
+  for (int m = 0; m < 3; ++m) {
+    const uint32_t p = (q.lo >> (8 * m)) & 0xFFu;
+    if (p != 0xFFu && p >= 31u) ++nbad;
+    w |= (uint64_t)(p == 0xFFu ? 31u : p & 31u) << (5 * m);
+  }
+  const uint32_t lv[3] = {(q.lo >> 24) | ((q.hi & 0xFu) << 8), (q.hi >> 4) & 0xFFFu,
+                          (q.hi >> 16) & 0xFFFu};
+  for (int o = 0; o < 3; ++o) {
+    if (lv[o] > 511u) ++nbad;
+    w |= (uint64_t)(lv[o] & 511u) << (15 + 9 * o);
+  }
+  const int64_t off = (int64_t)(q.el >> 4) - eb;
+  if (off < 0 || off >= (1 << 22)) ++nbad;
+  w |= (uint64_t)(off & ((1 << 22) - 1)) << 42;
+  return RecP1q{(uint32_t)w, (uint32_t)(w >> 32)};
+}
+
+// ELL copy of the incidence records (and P2 patch words) of every patch
+// block: one workgroup per block fills its slots (step k, lane t) from the
+// CSR-ordered records (layout: System::ell_rec), as they are (Out = Rec) or
+// in the RecP1q form (Out = RecP1q: ebase[b] = the block's lowest element,
+// slots of rows past the block's last or without incidences hold a zero
+// record, bad counts records that do not fit).
+template <typename Rec, typename Out>
+__global__ __launch_bounds__(256) void k_ell_fill(int64_t nvrows, int nvb, int64_t nrows,
+                                                  const int32_t *__restrict__ inc_ptr,
+                                                  const Rec *__restrict__ rec,
+                                                  const uint32_t *__restrict__ inc_lv,
+                                                  int64_t base_v, int steps_v, int lpr_v,
+                                                  int64_t base_e, int steps_e, int lpr_e,
+                                                  Out *__restrict__ ell,
+                                                  uint32_t *__restrict__ ell_lv,
+                                                  int32_t *__restrict__ ebase,
+                                                  int *__restrict__ bad) {
+  constexpr bool kPack = !std::is_same<Rec, Out>::value;
+  const int b = blockIdx.x;
+  int64_t r0, r1;
+  patch_block_rows(b, nvrows, nvb, nrows, r0, r1);
+  const bool vx = b < nvb;
+  const int lpr = vx ? lpr_v : lpr_e, steps = vx ? steps_v : steps_e;
+  const int nt = kPatchRows * lpr;
+  const int64_t e0 = (vx ? base_v : base_e) + (int64_t)(vx ? b : b - nvb) * steps * nt;
+  int eb = 0, nbad = 0;
+  if constexpr (kPack) {
+    __shared__ int emin;
+    if (threadIdx.x == 0) emin = INT_MAX;
+    __syncthreads();
+    int m = INT_MAX;
+    const int ia = r0 < r1 ? inc_ptr[r0] : 0, ib = r0 < r1 ? inc_ptr[r1] : 0;
+    for (int j = ia + (int)threadIdx.x; j < ib; j += blockDim.x) m = min(m, rec[j].el >> 4);
+    atomicMin(&emin, m);
+    __syncthreads();
+    eb = emin == INT_MAX ? 0 : emin;
+    if (threadIdx.x == 0) ebase[b] = eb;
+  }
+  for (int s = threadIdx.x; s < steps * nt; s += blockDim.x) {
+    const int k = s / nt, t = s - k * nt;
+    const int64_t r = r0 + t / lpr;
+    const int p = t % lpr + k * lpr;
+    int64_t src = -1;
+    if (r < r1) {
+      const int i0 = inc_ptr[r], n = inc_ptr[r + 1] - i0;
+      if (n > 0) src = i0 + (p < n ? p : n - 1);
+    }
+    if constexpr (kPack)
+      ell[e0 + s] = src < 0 ? Out{0u, 0u} : p1q_pack(rec[src], eb, nbad);
+    else
+      ell[e0 + s] = rec[src < 0 ? 0 : src];
+    if (ell_lv) ell_lv[e0 + s] = inc_lv[src < 0 ? 0 : src];
+  }
+  if (nbad) atomicAdd(bad, nbad);
+}
+
 // Vertex patches of a RecP1p / RecP2c system (vertex rows [0, nvrows), P2
 // edge rows after them); returns the patch size of every patch block (0: no
 // patch).  A block gets one when its distinct vertices fit the index form
@@ -687,6 +764,119 @@ static std::vector<int32_t> build_patches(Ctx *c, System &S, const std::vector<i
   return hc;
 }
 
+
+// Padded patch vertex ids (System::vid_pad): one workgroup per patch block.
+__global__ __launch_bounds__(256) void k_vid_pad(int nvb, int nb, const int32_t *__restrict__ pptr,
+                                                 const int32_t *__restrict__ vid,
+                                                 int64_t base_v, int stride_v, int64_t base_e,
+                                                 int stride_e, int32_t *__restrict__ out) {
+  const int b = blockIdx.x;
+  if (b >= nb) return;
+  const bool vx = b < nvb;
+  const int stride = vx ? stride_v : stride_e;
+  const int64_t o = (vx ? base_v : base_e) + (int64_t)(vx ? b : b - nvb) * stride;
+  const int p0 = pptr[b], np = pptr[b + 1] - p0;
+  for (int p = threadIdx.x; p < stride; p += blockDim.x)
+    out[o + p] = np > 0 ? vid[p0 + (p < np ? p : 0)] : 0;
+}
+
+// The ELL copy of a patched system's records (System::ell_*), laid out for
+// the lanes per row of its FP64 vertex-row and edge-row segments; P1 records
+// in the 8-byte RecP1q form where they fit (MGFE_P1Q=0: RecP1p).
+static void build_ell(Ctx *c, System &S, int64_t nvrows, const std::vector<int32_t> &psz) {
+  const size_t nb = psz.size();
+  int lpr[2] = {0, 0};
+  for (const RowSegment &sg : S.segs) {
+    if (sg.gacc) continue;  // accumulated in place: not patched
+    int &l = lpr[sg.lrow == 0 ? 0 : 1];
+    if (l == 0) l = sg.lpr;
+    MG_CHECK(l == sg.lpr, MG_ERR_STATE, "ell: segments of one kind with different lanes per row");
+  }
+  std::vector<int32_t> ip(S.nrows + 1);
+  MG_HIP(hipMemcpyAsync(ip.data(), S.inc_ptr, sizeof(int32_t) * (S.nrows + 1),
+                        hipMemcpyDeviceToHost, c->stream));
+  MG_HIP(hipStreamSynchronize(c->stream));
+  const int nvb = (int)((nvrows + kPatchRows - 1) / kPatchRows);
+  int steps[2] = {1, 1};
+  for (size_t b = 0; b < nb; ++b) {
+    const int g = (int)b < nvb ? 0 : 1;
+    const int l = std::max(1, lpr[g]);
+    const int64_t r0 = g == 0 ? (int64_t)b * kPatchRows : nvrows + (int64_t)(b - nvb) * kPatchRows;
+    const int64_t r1 = std::min<int64_t>(r0 + kPatchRows, g == 0 ? nvrows : S.nrows);
+    for (int64_t r = r0; r < r1; ++r) steps[g] = std::max(steps[g], (ip[r + 1] - ip[r] + l - 1) / l);
+  }
+  const int64_t nbv = std::min<int64_t>(nvb, (int64_t)nb), nbe = (int64_t)nb - nbv;
+  const int64_t sv = nbv * steps[0] * kPatchRows * std::max(1, lpr[0]);
+  const int64_t se = nbe * steps[1] * kPatchRows * std::max(1, lpr[1]);
+  MG_CHECK(sv + se < (1ll << 31), MG_ERR_SIZE, "ell: more than 2^31 slots");
+  const int64_t ns = std::max<int64_t>(sv + se, 1);
+  // 8-byte P1 records: rows of <= 31 entries, patches of <= 512 vertices
+  // (the element offsets are checked as the copy is filled)
+  int max_patch = 0;
+  for (int32_t n : psz) max_patch = std::max(max_patch, n);
+  if (S.rec_kind == kRecP1p && S.max_row_nnz <= 31 && max_patch <= 512 &&
+      env_int("MGFE_P1Q", 1) != 0 && nb > 0) {
+    S.ell_rec = dalloc<RecP1q>(c, ns);
+    S.ell_ebase = dalloc<int32_t>(c, nb);
+    int *bad = dalloc<int>(c, 1);
+    MG_HIP(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+    k_ell_fill<RecP1p, RecP1q><<<(unsigned)nb, 256, 0, c->stream>>>(
+        nvrows, nvb, S.nrows, S.inc_ptr, (const RecP1p *)S.inc_rec, nullptr, 0, steps[0],
+        std::max(1, lpr[0]), sv, steps[1], std::max(1, lpr[1]), (RecP1q *)S.ell_rec, nullptr,
+        S.ell_ebase, bad);
+    MG_HIP(hipGetLastError());
+    const int nbad = read_back(c, bad);
+    dfree(c, bad);
+    S.ell_q = nbad == 0;
+    if (!S.ell_q) {
+      dfree(c, S.ell_rec);
+      dfree(c, S.ell_ebase);
+      S.ell_rec = nullptr;
+      S.ell_ebase = nullptr;
+    }
+  }
+  if (!S.ell_q) {
+    S.ell_rec = dalloc<char>(c, (size_t)S.rec_words * 4 * ns);
+    if (S.inc_lv) S.ell_lv = dalloc<uint32_t>(c, ns);
+    if (nb > 0) {
+      if (S.rec_kind == kRecP1p)
+        k_ell_fill<RecP1p, RecP1p><<<(unsigned)nb, 256, 0, c->stream>>>(
+            nvrows, nvb, S.nrows, S.inc_ptr, (const RecP1p *)S.inc_rec, S.inc_lv, 0, steps[0],
+            std::max(1, lpr[0]), sv, steps[1], std::max(1, lpr[1]), (RecP1p *)S.ell_rec,
+            S.ell_lv, nullptr, nullptr);
+      else
+        k_ell_fill<RecP2c, RecP2c><<<(unsigned)nb, 256, 0, c->stream>>>(
+            nvrows, nvb, S.nrows, S.inc_ptr, (const RecP2c *)S.inc_rec, S.inc_lv, 0, steps[0],
+            std::max(1, lpr[0]), sv, steps[1], std::max(1, lpr[1]), (RecP2c *)S.ell_rec,
+            S.ell_lv, nullptr, nullptr);
+      MG_HIP(hipGetLastError());
+    }
+  }
+  MG_HIP(hipStreamSynchronize(c->stream));
+  S.ell_base[0] = 0;
+  S.ell_base[1] = sv;
+  for (int g = 0; g < 2; ++g) {
+    S.ell_steps[g] = steps[g];
+    S.ell_lpr[g] = lpr[g];
+  }
+  // padded patch vertex ids: per region, the largest number of vertices a
+  // launch of it stages
+  int stride[2] = {1, 1};
+  for (const RowSegment &sg : S.segs)
+    if (sg.patch > 0) stride[sg.lrow == 0 ? 0 : 1] = std::max(stride[sg.lrow == 0 ? 0 : 1], sg.patch);
+  const int64_t pv = nbv * stride[0], pe = nbe * stride[1];
+  S.vid_pad = dalloc<int32_t>(c, std::max<int64_t>(pv + pe, 1));
+  if (nb > 0) {
+    k_vid_pad<<<(unsigned)nb, 256, 0, c->stream>>>(nvb, (int)nb, S.patch_ptr, S.patch_vid, 0,
+                                                   stride[0], pv, stride[1], S.vid_pad);
+    MG_HIP(hipGetLastError());
+  }
+  MG_HIP(hipStreamSynchronize(c->stream));
+  S.vpad_base[0] = 0;
+  S.vpad_base[1] = pv;
+  S.vpad_stride[0] = stride[0];
+  S.vpad_stride[1] = stride[1];
+}
 
 // LDS budget of a staged row block: its entries plus the 256-slot region
 // that takes eliminated columns, in FP64, within 64 KiB
@@ -962,6 +1152,7 @@ void build_pattern(Ctx *c, int sys) {
   MG_CHECK(hb == 0, MG_ERR_STATE, "pattern: a record position does not fit its form");
   const int64_t nvrows =
       nl == 4 ? nrows : (sys == MG_SYS_FULL ? c->nV : c->nfree_vertex);
+  S.nvrows = nvrows;
   if (S.rec_kind == kRecP1p || (S.rec_kind == kRecP2c && patched)) {
     const std::vector<int32_t> psz = S.rec_kind == kRecP1p
                                          ? build_patches<RecP1p>(c, S, rp, nvrows)
@@ -993,6 +1184,11 @@ void build_pattern(Ctx *c, int sys) {
         S.edge_inc0 = i0;
       }
     }
+    // ELL records and padded patch ids: P1 systems (128^3 P1 FP64 assembly
+    // 310 -> 300 us); P2 rows measured no faster with them (MGFE_ELL=2: all
+    // patched systems, 0: none)
+    const int ell = env_int("MGFE_ELL", 1);
+    if (ell == 2 || (ell == 1 && nl == 4)) build_ell(c, S, nvrows, psz);
   } else {
     make_segments(S, rp, nvrows, nl, nullptr);
   }

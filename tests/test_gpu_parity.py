@@ -252,6 +252,45 @@ def test_edge_record_forms_bit_identical(ctx, monkeypatch, terms, rs, rm, prec, 
         assert np.abs(ref - ova).max() <= 1e-12 * np.abs(ova).max()
 
 
+@pytest.mark.parametrize("order,n", [(1, 9), (1, 14), (2, 5), (2, 7)])
+@pytest.mark.parametrize("terms,rs,rm", [("stiff", 4, 15), ("both", 1, 4)])
+@pytest.mark.parametrize("prec", [L.PREC_F64, L.PREC_F32])
+@pytest.mark.parametrize("sysid", [L.SYS_FULL, L.SYS_INTERIOR])
+def test_ell_records_bit_identical(ctx, monkeypatch, order, n, terms, rs, rm, prec, sysid):
+    # patched row blocks read their incidence records from an ELL copy laid
+    # out for their lanes per row and their patch vertex ids from a padded
+    # copy (MGFE_ELL=2: P1 and P2 systems, 1: P1 systems, 0: neither; other
+    # lanes per row: the CSR order), P1 records in the 8-byte form at four
+    # lanes per row (MGFE_P1Q=0: the 12-byte form): the same records and
+    # vertices in the same order, so the same bits
+    m = W.kuhn_cube(n, jitter=0.2, seed=37)
+    k = np.exp(W.stream_normal(0, W.STREAM_K, m.nT))
+    t = L.TERM_STIFF | (L.TERM_MASS if terms == "both" else 0)
+    if order == 2 and rs == 4:
+        rs = 11
+    out = []
+    for ell, lpr, q in (("0", None, "1"), ("2", None, "1"), ("1", None, "1"), ("1", None, "0"),
+                        ("2", "1", "1"), ("2", "2", "0")):
+        monkeypatch.setenv("MGFE_ELL", ell)
+        monkeypatch.setenv("MGFE_P1Q", q)
+        for var in ("MGFE_VERTEX_LPR", "MGFE_VERTEX_LPR32", "MGFE_EDGE_LPR"):
+            if lpr is None:
+                monkeypatch.delenv(var, raising=False)
+            else:
+                monkeypatch.setenv(var, lpr)
+        setup(ctx, m, order)
+        ctx.field_set(L.FIELD_STIFF, k)
+        ctx.assemble(sysid, t, stiff_coeff=L.COEFF_P0, stiff_rule=rs, mass_rule=rm, prec=prec)
+        out.append(ctx.values(sysid, prec=prec))
+    for o in out[1:]:
+        assert np.array_equal(out[0], o)
+    if prec == L.PREC_F64 and terms == "stiff":
+        orp, oci, ova = O.assemble(order, m, O.OP_STIFF, c_stiff=k, rule_stiff=4)
+        if sysid == L.SYS_INTERIOR:
+            orp, oci, ova = O.restrict(orp, oci, ova, O.dirichlet_mask(order, m, ALL_FACES))
+        assert (np.abs(out[1] - ova) / row_scale(orp, ova)).max() <= 1e-12
+
+
 @pytest.mark.parametrize("order", [1, 2])
 @pytest.mark.parametrize("prec", [L.PREC_F64, L.PREC_F32])
 @pytest.mark.parametrize("sysid", [L.SYS_FULL, L.SYS_INTERIOR])
