@@ -467,20 +467,24 @@ def test_pcg_reports_nonconvergence(ctx):
         ctx.pcg(b, mode=L.SOLVE_F64, tol=1e-10, max_iters=3)
 
 
-@pytest.mark.parametrize("order", [1, 2])
-def test_adjoint_kernel_matches_oracle(ctx, order):
+# rules exact on the stiffness integrand take the closed form (P1: all; P2:
+# 4, 11, 15 points), the P2 1-point rule the quadrature-table form
+@pytest.mark.parametrize("order,rule", [(1, 4), (1, 1), (2, 4), (2, 11), (2, 1)])
+@pytest.mark.parametrize("sysid", [L.SYS_FULL, L.SYS_INTERIOR])
+def test_adjoint_kernel_matches_oracle(ctx, order, rule, sysid):
     m = W.kuhn_cube(4, jitter=0.2, seed=11)
     setup(ctx, m, order)
     fixed = ctx.dirichlet_mask()
-    nf = int((fixed == 0).sum())
+    free = np.ones(fixed.size, bool) if sysid == L.SYS_FULL else fixed == 0
+    nf = int(free.sum())
     u = W.stream_uniform(0, 7, nf) - 0.5
     lam = W.stream_uniform(1, 7, nf) - 0.5
-    g = ctx.adjoint_grad_k(L.SYS_INTERIOR, u, lam, rule=4)
+    g = ctx.adjoint_grad_k(sysid, u, lam, rule=rule)
     uf = np.zeros(fixed.size)
     lf = np.zeros(fixed.size)
-    uf[fixed == 0] = u
-    lf[fixed == 0] = lam
-    go = O.adjoint_grad_k(order, m, ctx.elem_dofs(), uf, lf, rule=4)
+    uf[free] = u
+    lf[free] = lam
+    go = O.adjoint_grad_k(order, m, ctx.elem_dofs(), uf, lf, rule=rule)
     assert np.abs(g - go).max() <= 1e-12 * np.abs(go).max()
 
 
