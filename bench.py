@@ -460,9 +460,16 @@ def main():
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the 96^3 drift-diffusion and 128^3 gradient extras")
+    ap.add_argument("--dd-only", action="store_true",
+                    help="the 96^3 drift-diffusion residual and VJP only (--steps repetitions "
+                         "each), printed as one JSON line")
     args = ap.parse_args()
     if args.traffic_child:
         traffic_child(args.n)
+        return
+    if args.dd_only:
+        ctx = L.Context(0)
+        print(json.dumps(bench_config4(ctx, HipTimer(ctx.stream), n=96, reps=args.steps)))
         return
     if args.assembly_only:
         args.no_solve = args.no_cpu = True
