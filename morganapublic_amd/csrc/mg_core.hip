@@ -254,13 +254,13 @@ static void reset_derived(Ctx *c) {
   for (void *p : {(void *)c->edge_keys, (void *)c->edge_nodes, (void *)c->elem_edge,
                   (void *)c->dofs, (void *)c->fixed, (void *)c->sysmap, (void *)c->edge_surf,
                   (void *)c->edge_n, (void *)c->cell_volume, (void *)c->sigma_el,
-                  (void *)c->vinc_ptr, (void *)c->vinc, (void *)c->vinc_pos,
-                  (void *)c->dd_contrib})
+                  (void *)c->sigma_len, (void *)c->vinc_ptr, (void *)c->vinc,
+                  (void *)c->vinc_pos, (void *)c->dd_contrib})
     dfree(c, p);
   c->edge_keys = nullptr;
   c->edge_nodes = c->elem_edge = c->dofs = c->sysmap = nullptr;
   c->fixed = nullptr;
-  c->edge_surf = c->edge_n = c->cell_volume = c->sigma_el = nullptr;
+  c->edge_surf = c->edge_n = c->cell_volume = c->sigma_el = c->sigma_len = nullptr;
   c->vinc_ptr = c->vinc = nullptr;
   c->dd_contrib = nullptr;
   c->vinc_pos = nullptr;

@@ -121,6 +121,28 @@ __global__ void k_sigma(int64_t nT, const int32_t *__restrict__ elem_edge,
   sigma[i] = 0.5 * (ne[3 * i] * nh[0] + ne[3 * i + 1] * nh[1] + ne[3 * i + 2] * nh[2]);
 }
 
+// sigma_len[6e + j] = sigma_el[6e + j] / |edge j of e|: the only form in
+// which the SG fluxes use the dual areas, so the residual kernels need no
+// edge lengths (no square roots or divisions per edge)
+__global__ void k_sigma_len(int64_t nT, const int32_t *__restrict__ tets,
+                            const double *__restrict__ xyz, const double *__restrict__ sigma,
+                            double *__restrict__ sigma_len) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nT) return;
+  const int4 t = reinterpret_cast<const int4 *>(tets)[e];
+  const int32_t id[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    double s2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double tt = xyz[3 * (size_t)id[kEA[j]] + d] - xyz[3 * (size_t)id[kEB[j]] + d];
+      s2 += tt * tt;
+    }
+    sigma_len[6 * e + j] = sigma[6 * e + j] / sqrt(s2);
+  }
+}
+
 __global__ void k_vertex_gather(int64_t nV, const int32_t *__restrict__ ptr,
                                 const int32_t *__restrict__ inc, const double *__restrict__ subvol,
                                 double *__restrict__ cvol) {
@@ -201,17 +223,21 @@ void build_dual(Ctx *c) {
   int32_t *eptr = nullptr, *einc = nullptr;
   group_by(c, c->elem_edge, 6 * nT, nE, 0, &eptr, &einc);
   build_vertex_incidence(c);
-  for (double *p : {c->edge_surf, c->edge_n, c->cell_volume, c->sigma_el, c->dd_contrib})
+  for (double *p : {c->edge_surf, c->edge_n, c->cell_volume, c->sigma_el, c->sigma_len,
+                    c->dd_contrib})
     dfree(c, p);
   c->edge_surf = dalloc<double>(c, nE);
   c->edge_n = dalloc<double>(c, 3 * nE);
   c->cell_volume = dalloc<double>(c, nV);
   c->sigma_el = dalloc<double>(c, 6 * nT);
+  c->sigma_len = dalloc<double>(c, 6 * nT);
   c->dd_contrib = dalloc<double>(c, 8 * nT);
   k_edge_gather<<<blocks_for(nE, 256), 256, 0, c->stream>>>(nE, eptr, einc, ne, c->edge_surf,
                                                           c->edge_n);
   k_sigma<<<blocks_for(6 * nT, 256), 256, 0, c->stream>>>(nT, c->elem_edge, ne, c->edge_n,
                                                         c->sigma_el);
+  k_sigma_len<<<blocks_for(nT, 256), 256, 0, c->stream>>>(nT, c->tets, c->xyz, c->sigma_el,
+                                                        c->sigma_len);
   k_vertex_gather<<<blocks_for(nV, 256), 256, 0, c->stream>>>(nV, c->vinc_ptr, c->vinc, sv,
                                                             c->cell_volume);
   MG_HIP(hipGetLastError());
@@ -265,25 +291,19 @@ __device__ __forceinline__ double p1_geometry(const double P[4][3], double G[4][
   return d;
 }
 
-// SG flux of local edge (a, b) (a, b compile-time) with coefficient coef,
-// oriented from a to b: coef / L (n_b B(phi_b - phi_a) - n_a B(phi_a - phi_b)).
+// SG flux of local edge (a, b) (a, b compile-time) with coefficient coef
+// (mu sigma / L: the length is in sigma_len), oriented from a to b:
+// coef (n_b B(phi_b - phi_a) - n_a B(phi_a - phi_b)).
 // One Bernoulli evaluation per edge: with t = |x|, B(-t) = t + B(t) (both
 // terms positive, no cancellation).
-__device__ __forceinline__ double sg_edge_flux(const double P[4][3], const double ph[4],
-                                               const double dn[4], int a, int b, double coef) {
-  double s2 = 0.0;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double tt = P[a][d] - P[b][d];
-    s2 += tt * tt;
-  }
-  const double L = sqrt(s2);
+__device__ __forceinline__ double sg_edge_flux(const double ph[4], const double dn[4], int a,
+                                               int b, double coef) {
   const double dab = ph[b] - ph[a];
   const double t = fabs(dab);
   const double bt = bernoulli(t), bmt = t + bt;  // B(t), B(-t)
   const double bp = dab >= 0.0 ? bt : bmt;       // B(dab)
   const double bm = dab >= 0.0 ? bmt : bt;       // B(-dab)
-  return coef / L * (dn[b] * bp - dn[a] * bm);
+  return coef * (dn[b] * bp - dn[a] * bm);
 }
 
 // The residual in two passes.  k_dd_elem: one lane per element computes its
@@ -298,7 +318,7 @@ __device__ __forceinline__ double sg_edge_flux(const double P[4][3], const doubl
 // writes its partial of the loss sum (r_phi^2 + r_n^2).
 __global__ __launch_bounds__(128) void k_dd_elem(
     int64_t nT, const int32_t *__restrict__ tets, const double *__restrict__ xyz,
-    const double *__restrict__ sigma, const double *__restrict__ eps,
+    const double *__restrict__ sigma_len, const double *__restrict__ eps,
     const double *__restrict__ rho, const double *__restrict__ mu, const double *__restrict__ phi,
     const double *__restrict__ dens, const int32_t *__restrict__ pos,
     double2 *__restrict__ contrib) {
@@ -317,7 +337,7 @@ __global__ __launch_bounds__(128) void k_dd_elem(
     dn[q] = dens[id[q]];
   }
 #pragma unroll
-  for (int j = 0; j < 6; ++j) sg[j] = sigma[6 * e + j];
+  for (int j = 0; j < 6; ++j) sg[j] = sigma_len[6 * e + j];
   const double ee = eps[e], re = rho[e], me = mu[e];
   // Poisson terms first, so the geometry is dead during the SG part (100
   // VGPRs, 4 waves per SIMD, against 148 with the geometry kept live)
@@ -339,7 +359,7 @@ __global__ __launch_bounds__(128) void k_dd_elem(
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
     const int a = kEA[j], b = kEB[j];
-    const double F = sg_edge_flux(P, ph, dn, a, b, me * sg[j]);
+    const double F = sg_edge_flux(ph, dn, a, b, me * sg[j]);
     cn[a] += F;
     cn[b] -= F;
   }
@@ -391,7 +411,7 @@ __global__ __launch_bounds__(256) void k_dd_gather(int64_t nV, const int32_t *__
 
 __global__ __launch_bounds__(128) void k_dd_vjp(
     int64_t nT, const int32_t *__restrict__ tets, const double *__restrict__ xyz,
-    const double *__restrict__ sigma, const double *__restrict__ phi,
+    const double *__restrict__ sigma_len, const double *__restrict__ phi,
     const double *__restrict__ dens, const double *__restrict__ rphi,
     const double *__restrict__ rn, double *__restrict__ g_eps, double *__restrict__ g_mu) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -409,7 +429,7 @@ __global__ __launch_bounds__(128) void k_dd_vjp(
     rv[q] = rn[id[q]];
   }
 #pragma unroll
-  for (int j = 0; j < 6; ++j) sg[j] = sigma[6 * e + j];
+  for (int j = 0; j < 6; ++j) sg[j] = sigma_len[6 * e + j];
   double ge = 0.0;
   {
     double G[4][3];
@@ -427,7 +447,7 @@ __global__ __launch_bounds__(128) void k_dd_vjp(
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
     const int a = kEA[j], b = kEB[j];
-    gm += 2.0 * (rv[a] - rv[b]) * sg_edge_flux(P, ph, dn, a, b, sg[j]);
+    gm += 2.0 * (rv[a] - rv[b]) * sg_edge_flux(ph, dn, a, b, sg[j]);
   }
   g_eps[e] = ge;
   g_mu[e] = gm;
@@ -495,7 +515,7 @@ static void dd_upload(Ctx *c, DDHost &B, const double *eps, const double *rho, c
 // block partials go to B.partial (dd_gather_blocks entries)
 static void dd_residual_launch(Ctx *c, DDBuffers &B, double *loss = nullptr) {
   double2 *contrib = reinterpret_cast<double2 *>(c->dd_contrib);
-  k_dd_elem<<<blocks_for(c->nT, 128), 128, 0, c->stream>>>(c->nT, c->tets, c->xyz, c->sigma_el,
+  k_dd_elem<<<blocks_for(c->nT, 128), 128, 0, c->stream>>>(c->nT, c->tets, c->xyz, c->sigma_len,
                                                           B.eps, B.rho, B.mu, B.phi, B.dens,
                                                           c->vinc_pos, contrib);
   const int gb = dd_gather_blocks(c);
@@ -583,7 +603,7 @@ int mg_dd_vjp(mg_ctx *c, const double *eps, const double *rho, const double *mu,
   DDHost B(c);
   dd_upload(c, B, eps, rho, mu, phi, dens, fixed);
   dd_residual_launch(c, B);
-  k_dd_vjp<<<blocks_for(c->nT, 128), 128, 0, c->stream>>>(c->nT, c->tets, c->xyz, c->sigma_el,
+  k_dd_vjp<<<blocks_for(c->nT, 128), 128, 0, c->stream>>>(c->nT, c->tets, c->xyz, c->sigma_len,
                                                           B.phi, B.dens, B.rphi, B.rn, B.ge, B.gm);
   MG_HIP(hipGetLastError());
   MG_HIP(hipMemcpyAsync(g_eps, B.ge, sizeof(double) * c->nT, hipMemcpyDeviceToHost, c->stream));
@@ -618,7 +638,7 @@ int mg_dd_vjp_device(mg_ctx *c, const double *eps, const double *rho, const doub
               const_cast<double *>(phi), const_cast<double *>(dens), r_phi, r_n, g_eps, g_mu,
               nullptr, const_cast<uint8_t *>(fixed)};
   dd_residual_launch(c, B);
-  k_dd_vjp<<<blocks_for(c->nT, 128), 128, 0, c->stream>>>(c->nT, c->tets, c->xyz, c->sigma_el,
+  k_dd_vjp<<<blocks_for(c->nT, 128), 128, 0, c->stream>>>(c->nT, c->tets, c->xyz, c->sigma_len,
                                                           B.phi, B.dens, B.rphi, B.rn, B.ge, B.gm);
   MG_HIP(hipGetLastError());
   API_END(c)

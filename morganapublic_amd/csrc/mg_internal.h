@@ -264,10 +264,11 @@ struct Ctx {
   // dual mesh
   bool dual_ok = false;
   double *edge_surf = nullptr, *edge_n = nullptr, *cell_volume = nullptr, *sigma_el = nullptr;
+  double *sigma_len = nullptr;  // 6 nT: sigma_el over the edge's length (the SG coefficient's geometry)
   int32_t *vinc_ptr = nullptr;  // vertex -> (element, local vertex) incidence
   int32_t *vinc = nullptr;
   int32_t *vinc_pos = nullptr;   // 4 nT: incidence (4e + l) -> its index in vinc
-  double *dd_contrib = nullptr;  // 8 nT: drift-diffusion residual terms in vinc order  // 8 nT: drift-diffusion residual terms per (element, local vertex)
+  double *dd_contrib = nullptr;  // 8 nT: drift-diffusion residual terms in vinc order
 
   // scratch for library primitives (sorts, scans)
   void *tmp = nullptr;
