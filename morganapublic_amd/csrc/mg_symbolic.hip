@@ -633,8 +633,9 @@ __global__ void k_edge5(int64_t i0, int64_t n_inc, RecP2c *__restrict__ rec,
 }
 
 // RecP1q form of a patched P1 record, its element taken relative to eb (the
-// lowest element of its block); nbad counts the fields that do not fit
-__device__ __forceinline__ RecP1q p1q_pack(const RecP1p &q, int eb, int &nbad) {
+// lowest element of its block) and accepted below 2^span (span <= 22, the
+// field's width); nbad counts the fields that do not fit
+__device__ __forceinline__ RecP1q p1q_pack(const RecP1p &q, int eb, int span, int &nbad) {
   uint64_t w = 0;
   for (int m = 0; m < 3; ++m) {
     const uint32_t p = (q.lo >> (8 * m)) & 0xFFu;
@@ -648,7 +649,7 @@ __device__ __forceinline__ RecP1q p1q_pack(const RecP1p &q, int eb, int &nbad) {
     w |= (uint64_t)(lv[o] & 511u) << (15 + 9 * o);
   }
   const int64_t off = (int64_t)(q.el >> 4) - eb;
-  if (off < 0 || off >= (1 << 22)) ++nbad;
+  if (off < 0 || off >= (1 << span)) ++nbad;
   w |= (uint64_t)(off & ((1 << 22) - 1)) << 42;
   return RecP1q{(uint32_t)w, (uint32_t)(w >> 32)};
 }
@@ -668,7 +669,7 @@ __global__ __launch_bounds__(256) void k_ell_fill(int64_t nvrows, int nvb, int64
                                                   int64_t base_e, int steps_e, int lpr_e,
                                                   Out *__restrict__ ell,
                                                   uint32_t *__restrict__ ell_lv,
-                                                  int32_t *__restrict__ ebase,
+                                                  int32_t *__restrict__ ebase, int span,
                                                   int *__restrict__ bad) {
   constexpr bool kPack = !std::is_same<Rec, Out>::value;
   const int b = blockIdx.x;
@@ -701,7 +702,7 @@ __global__ __launch_bounds__(256) void k_ell_fill(int64_t nvrows, int nvb, int64
       if (n > 0) src = i0 + (p < n ? p : n - 1);
     }
     if constexpr (kPack)
-      ell[e0 + s] = src < 0 ? Out{0u, 0u} : p1q_pack(rec[src], eb, nbad);
+      ell[e0 + s] = src < 0 ? Out{0u, 0u} : p1q_pack(rec[src], eb, span, nbad);
     else
       ell[e0 + s] = rec[src < 0 ? 0 : src];
     if (ell_lv) ell_lv[e0 + s] = inc_lv[src < 0 ? 0 : src];
@@ -811,7 +812,8 @@ static void build_ell(Ctx *c, System &S, int64_t nvrows, const std::vector<int32
   MG_CHECK(sv + se < (1ll << 31), MG_ERR_SIZE, "ell: more than 2^31 slots");
   const int64_t ns = std::max<int64_t>(sv + se, 1);
   // 8-byte P1 records: rows of <= 31 entries, patches of <= 512 vertices
-  // (the element offsets are checked as the copy is filled)
+  // (the element offsets are checked as the copy is filled, against 2^22 or
+  // the smaller MGFE_P1Q_SPAN bits, which tests use to force the fallback)
   int max_patch = 0;
   for (int32_t n : psz) max_patch = std::max(max_patch, n);
   if (S.rec_kind == kRecP1p && S.max_row_nnz <= 31 && max_patch <= 512 &&
@@ -823,7 +825,7 @@ static void build_ell(Ctx *c, System &S, int64_t nvrows, const std::vector<int32
     k_ell_fill<RecP1p, RecP1q><<<(unsigned)nb, 256, 0, c->stream>>>(
         nvrows, nvb, S.nrows, S.inc_ptr, (const RecP1p *)S.inc_rec, nullptr, 0, steps[0],
         std::max(1, lpr[0]), sv, steps[1], std::max(1, lpr[1]), (RecP1q *)S.ell_rec, nullptr,
-        S.ell_ebase, bad);
+        S.ell_ebase, std::min(22, std::max(0, env_int("MGFE_P1Q_SPAN", 22))), bad);
     MG_HIP(hipGetLastError());
     const int nbad = read_back(c, bad);
     dfree(c, bad);
@@ -843,12 +845,12 @@ static void build_ell(Ctx *c, System &S, int64_t nvrows, const std::vector<int32
         k_ell_fill<RecP1p, RecP1p><<<(unsigned)nb, 256, 0, c->stream>>>(
             nvrows, nvb, S.nrows, S.inc_ptr, (const RecP1p *)S.inc_rec, S.inc_lv, 0, steps[0],
             std::max(1, lpr[0]), sv, steps[1], std::max(1, lpr[1]), (RecP1p *)S.ell_rec,
-            S.ell_lv, nullptr, nullptr);
+            S.ell_lv, nullptr, 0, nullptr);
       else
         k_ell_fill<RecP2c, RecP2c><<<(unsigned)nb, 256, 0, c->stream>>>(
             nvrows, nvb, S.nrows, S.inc_ptr, (const RecP2c *)S.inc_rec, S.inc_lv, 0, steps[0],
             std::max(1, lpr[0]), sv, steps[1], std::max(1, lpr[1]), (RecP2c *)S.ell_rec,
-            S.ell_lv, nullptr, nullptr);
+            S.ell_lv, nullptr, 0, nullptr);
       MG_HIP(hipGetLastError());
     }
   }

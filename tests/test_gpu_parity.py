@@ -269,10 +269,16 @@ def test_ell_records_bit_identical(ctx, monkeypatch, order, n, terms, rs, rm, pr
     if order == 2 and rs == 4:
         rs = 11
     out = []
+    # (MGFE_P1Q_SPAN=2: element offsets past 3 do not fit the 8-byte form,
+    # so the copy falls back to 12-byte records after trying it)
     for ell, lpr, q in (("0", None, "1"), ("2", None, "1"), ("1", None, "1"), ("1", None, "0"),
-                        ("2", "1", "1"), ("2", "2", "0")):
+                        ("2", "1", "1"), ("2", "2", "0"), ("1", None, "span")):
         monkeypatch.setenv("MGFE_ELL", ell)
-        monkeypatch.setenv("MGFE_P1Q", q)
+        monkeypatch.setenv("MGFE_P1Q", "0" if q == "0" else "1")
+        if q == "span":
+            monkeypatch.setenv("MGFE_P1Q_SPAN", "2")
+        else:
+            monkeypatch.delenv("MGFE_P1Q_SPAN", raising=False)
         for var in ("MGFE_VERTEX_LPR", "MGFE_VERTEX_LPR32", "MGFE_EDGE_LPR"):
             if lpr is None:
                 monkeypatch.delenv(var, raising=False)
