@@ -29,8 +29,11 @@ namespace mg {
 static constexpr int kThreads = 256;
 static constexpr int kMaxBlocks = 2048;  // SpMV grid cap = size of every partial array
 // vector-kernel grid cap: every workgroup of k_update / k_pupdate re-totals the
-// previous kernel's partials, so fewer, fuller workgroups read less of them
-static constexpr int kMaxVecBlocks = 1024;
+// previous kernel's partials, so fewer, fuller workgroups read less of them.
+// One workgroup per CU of an MI355X (64^3 P2 interior, against 1024: FP64
+// PCG 179.4 -> 177.0 ms, FP32 with FP64 residual replacement 143.5 -> 140.1
+// ms; 128 or 384: slower)
+static constexpr int kMaxVecBlocks = 256;
 
 // Host-visible solver state; written by block 0 of a kernel or by a one-block
 // finalize kernel, read by later kernels (kernel boundaries order the writes).
