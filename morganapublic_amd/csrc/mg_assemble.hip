@@ -269,6 +269,9 @@ struct PatchSrc {
   int vpad_stride;
   // RecP1q records: the lowest element of each patch block
   const int32_t *ebase;
+  // packed RecP2e edge-row words (System::erow) and the first edge row
+  const uint64_t *erow;
+  int64_t erow_r0;
 };
 
 struct ElemSrc {
@@ -646,17 +649,30 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
   constexpr bool kRecLv = std::is_same<Rec, RecP2e>::value;
   using RV = RecView<Rec, LROW>;
   int base = 0, iend = 0, i = 0, self = 0;
+  uint32_t lvr = 0;
+  // RecP2e edge rows with packed row words: one 8-byte load per lane
+  const bool erow = PATCH && kRecLv && pc.erow != nullptr;
   if (kChunked) {
-    base = rowptr[rr] - s0;
-    iend = inc_ptr[rr + 1];
-    i = inc_ptr[rr] + sub;
-    if (kSelf) self = diagpos[rr] - s0 - base;  // position of the diagonal in the row
+    if (erow) {
+      const int inc0 = inc_ptr[r0];
+      const uint64_t w = pc.erow[rr - pc.erow_r0];
+      const int i0 = inc0 + (int)((w >> 21) & 2047u);
+      base = (int)(w & 0xFFFFu);
+      self = (int)((w >> 16) & 31u);
+      iend = i0 + (int)((w >> 32) & 63u);
+      i = i0 + sub;
+      lvr = (uint32_t)((w >> 38) & 511u) | ((uint32_t)((w >> 47) & 511u) << 16);
+    } else {
+      base = rowptr[rr] - s0;
+      iend = inc_ptr[rr + 1];
+      i = inc_ptr[rr] + sub;
+      if (kSelf) self = diagpos[rr] - s0 - base;  // position of the diagonal in the row
+    }
   }
   // vertex patch: coordinates as three arrays of pc.slots doubles after the
   // staged entries; npm = the patch's last index (record indices are clamped
   // to it: a row's records always point into its own block's patch)
   double *PX = nullptr;
-  uint32_t lvr = 0;
   int npm = 0, p0 = 0;
   // coordinates of patch vertex lv (staged, or past the staged ones): the
   // LDS read is unconditional (clamped index) and only the memory read sits
@@ -731,7 +747,7 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
     // larger patch (a rare outlier: the launch's LDS is sized for its
     // typical blocks) reads the coordinates of the others from memory
     const int ns_p = kOvf ? min(np, pc.slots) : np;
-    lvr = pc.row_lv[rr];
+    if (!erow) lvr = pc.row_lv[rr];
     // up to three vertices per lane with their loads in flight together;
     // from the padded ids (all pc.slots of them, whatever the block's patch
     // size, so the loads do not wait for it) or the block's own
@@ -1160,7 +1176,8 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
                 S.ell_base[g] + q0 * S.ell_steps[g] * kPatchRows * std::max(1, S.ell_lpr[g]),
                 S.ell_steps[g], 0,
                 S.vid_pad && sg.patch <= S.vpad_stride[g] ? S.vid_pad : nullptr,
-                S.vpad_base[g] + q0 * S.vpad_stride[g], S.vpad_stride[g], S.ell_ebase};
+                S.vpad_base[g] + q0 * S.vpad_stride[g], S.vpad_stride[g], S.ell_ebase,
+                S.erow, S.nvrows};
     auto go = [&](auto lr, auto ms, auto ex, auto rw, auto lp, auto ga, auto pt) {
       // the grid and the LDS size come from the segment's block rows
       MG_CHECK(decltype(rw)::value == sg.rows, MG_ERR_STATE,

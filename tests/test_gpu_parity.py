@@ -226,21 +226,24 @@ def test_assembly_launch_shapes_bit_identical(ctx, monkeypatch, order, rs, prec,
 def test_edge_record_forms_bit_identical(ctx, monkeypatch, terms, rs, rm, prec, sysid):
     # P2 edge-dof rows read 12-byte records that carry their patch words
     # (5-bit positions) or 12-byte records plus separate 4-byte patch words
-    # (7-bit positions): the same positions and patch indices, so the same
-    # bits, for every lanes-per-row shape; the oracle pins the default form
+    # (7-bit positions), and their row data from one packed word per row or
+    # from the separate row arrays (MGFE_EROW=0): the same positions, patch
+    # indices and ranges, so the same bits, for every lanes-per-row shape;
+    # the oracle pins the default form
     m = W.kuhn_cube(6, jitter=0.2, seed=23)
     k = np.exp(W.stream_normal(0, W.STREAM_K, m.nT))
     t = L.TERM_STIFF | (L.TERM_MASS if terms == "both" else 0)
     out = {}
-    for e5 in ("1", "0"):
+    for e5, er in (("1", "1"), ("1", "0"), ("0", "1")):
         for lpr in ("2", "1"):
             monkeypatch.setenv("MGFE_EDGE5", e5)
+            monkeypatch.setenv("MGFE_EROW", er)
             monkeypatch.setenv("MGFE_EDGE_LPR", lpr)
             setup(ctx, m, 2)
             ctx.field_set(L.FIELD_STIFF, k)
             ctx.assemble(sysid, t, stiff_coeff=L.COEFF_P0, stiff_rule=rs, mass_rule=rm, prec=prec)
-            out[(e5, lpr)] = ctx.values(sysid, prec=prec)
-    ref = out[("1", "2")]
+            out[(e5, er, lpr)] = ctx.values(sysid, prec=prec)
+    ref = out[("1", "1", "2")]
     for key, v in out.items():
         assert np.array_equal(ref, v), key
     if prec == L.PREC_F64 and terms == "stiff":
