@@ -269,9 +269,8 @@ struct PatchSrc {
   int vpad_stride;
   // RecP1q records: the lowest element of each patch block
   const int32_t *ebase;
-  // packed RecP2e edge-row words (System::erow) and the first edge row
-  const uint64_t *erow;
-  int64_t erow_r0;
+  // packed row words (System::prow), or null
+  const uint64_t *prow;
 };
 
 struct ElemSrc {
@@ -650,18 +649,19 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
   using RV = RecView<Rec, LROW>;
   int base = 0, iend = 0, i = 0, self = 0;
   uint32_t lvr = 0;
-  // RecP2e edge rows with packed row words: one 8-byte load per lane
-  const bool erow = PATCH && kRecLv && pc.erow != nullptr;
+  // patched rows with packed row words: one 8-byte load per lane (64^3 P2
+  // FP64 assembly -1.3% from the edge rows' words alone)
+  const bool prow = PATCH && pc.prow != nullptr;
   if (kChunked) {
-    if (erow) {
+    if (prow) {
       const int inc0 = inc_ptr[r0];
-      const uint64_t w = pc.erow[rr - pc.erow_r0];
-      const int i0 = inc0 + (int)((w >> 21) & 2047u);
+      const uint64_t w = pc.prow[rr];
+      const int i0 = inc0 + (int)((w >> 24) & 4095u);
       base = (int)(w & 0xFFFFu);
-      self = (int)((w >> 16) & 31u);
-      iend = i0 + (int)((w >> 32) & 63u);
+      self = (int)((w >> 16) & 255u);
+      iend = i0 + (int)((w >> 36) & 255u);
       i = i0 + sub;
-      lvr = (uint32_t)((w >> 38) & 511u) | ((uint32_t)((w >> 47) & 511u) << 16);
+      lvr = (uint32_t)((w >> 44) & 1023u) | ((uint32_t)((w >> 54) & 1023u) << 16);
     } else {
       base = rowptr[rr] - s0;
       iend = inc_ptr[rr + 1];
@@ -747,7 +747,7 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
     // larger patch (a rare outlier: the launch's LDS is sized for its
     // typical blocks) reads the coordinates of the others from memory
     const int ns_p = kOvf ? min(np, pc.slots) : np;
-    if (!erow) lvr = pc.row_lv[rr];
+    if (!prow) lvr = pc.row_lv[rr];
     // up to three vertices per lane with their loads in flight together;
     // from the padded ids (all pc.slots of them, whatever the block's patch
     // size, so the loads do not wait for it) or the block's own
@@ -1177,7 +1177,7 @@ static void launch_rows(Ctx *c, System &S, const mg_operator *op, T *vals, bool 
                 S.ell_steps[g], 0,
                 S.vid_pad && sg.patch <= S.vpad_stride[g] ? S.vid_pad : nullptr,
                 S.vpad_base[g] + q0 * S.vpad_stride[g], S.vpad_stride[g], S.ell_ebase,
-                S.erow, S.nvrows};
+                S.prow};
     auto go = [&](auto lr, auto ms, auto ex, auto rw, auto lp, auto ga, auto pt) {
       // the grid and the LDS size come from the segment's block rows
       MG_CHECK(decltype(rw)::value == sg.rows, MG_ERR_STATE,

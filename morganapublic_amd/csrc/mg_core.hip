@@ -25,7 +25,7 @@ void free_system(Ctx *c, System &S) {
                   (void *)S.dinv64, (void *)S.dinv32, (void *)S.sdesc, (void *)S.cofs,
                   (void *)S.long_rows, (void *)S.patch_ptr, (void *)S.patch_vid,
                   (void *)S.row_lv, (void *)S.inc_lv, S.ell_rec, (void *)S.vid_pad,
-                  (void *)S.ell_lv, (void *)S.ell_ebase, (void *)S.erow})
+                  (void *)S.ell_lv, (void *)S.ell_ebase, (void *)S.prow})
     dfree(c, p);
   S = System();
 }

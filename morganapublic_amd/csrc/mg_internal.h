@@ -168,13 +168,13 @@ struct System {
   // in the RecP2e form, which carries their patch words
   bool edge5 = false;
   int64_t edge_inc0 = 0;
-  // one packed word per RecP2e edge row (index r - nvrows): the row's value
-  // offset in its kPatchRows-row block (bits 0-15), its diagonal's position
-  // in the row (16-20), its first incidence relative to the block's first
-  // (21-31), its incidence count (32-37) and its endpoints' patch indices
-  // (38-46, 47-55): one load in place of rowptr, inc_ptr (2), diagpos and
-  // row_lv (null: a row did not fit, the separate arrays are read)
-  uint64_t *erow = nullptr;
+  // one packed word per row of a patched system: the row's value offset in
+  // its kPatchRows-row patch block (bits 0-15), its diagonal's position in
+  // the row (16-23), its first incidence relative to the block's first
+  // (24-35), its incidence count (36-43) and the patch indices of its vertex
+  // or endpoints (44-53, 54-63): one load in place of rowptr, inc_ptr (2),
+  // diagpos and row_lv (null: a row did not fit, the arrays are read)
+  uint64_t *prow = nullptr;
   // ELL copy of the patch blocks' incidence records, one region for the
   // vertex-dof blocks (0) and one for the P2 edge-dof blocks (1), each with
   // ell_steps[g] steps per block: slot (step k, lane t) of the region's block

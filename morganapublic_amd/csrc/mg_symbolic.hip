@@ -632,25 +632,26 @@ __global__ void k_edge5(int64_t i0, int64_t n_inc, RecP2c *__restrict__ rec,
   reinterpret_cast<RecP2e *>(rec)[i] = RecP2e{q.el, (uint32_t)w, (uint32_t)(w >> 32)};
 }
 
-// Packed words of the RecP2e edge rows [nvrows, nrows) (System::erow); bad
-// counts rows whose fields do not fit.
-__global__ void k_erow(int64_t nvrows, int64_t nrows, const int32_t *__restrict__ rowptr,
+// Packed row words of a patched system (System::prow); bad counts rows
+// whose fields do not fit.
+__global__ void k_prow(int64_t nvrows, int64_t nrows, const int32_t *__restrict__ rowptr,
                        const int32_t *__restrict__ inc_ptr, const int32_t *__restrict__ diagpos,
-                       const uint32_t *__restrict__ row_lv, uint64_t *__restrict__ erow,
+                       const uint32_t *__restrict__ row_lv, uint64_t *__restrict__ prow,
                        int *__restrict__ bad) {
-  const int64_t r = nvrows + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
-  const int64_t r0 = nvrows + ((r - nvrows) / kPatchRows) * kPatchRows;
+  const int64_t r0 = r < nvrows ? (r / kPatchRows) * kPatchRows
+                                : nvrows + ((r - nvrows) / kPatchRows) * kPatchRows;
   const int vo = rowptr[r] - rowptr[r0];
   const int io = inc_ptr[r] - inc_ptr[r0];
   const int cnt = inc_ptr[r + 1] - inc_ptr[r];
   const int self = diagpos[r] - rowptr[r];
   const uint32_t a = row_lv[r] & 0xFFFFu, b = row_lv[r] >> 16;
-  const bool ok = vo >= 0 && vo <= 0xFFFF && io >= 0 && io <= 2047 && cnt >= 0 && cnt <= 63 &&
-                  self >= 0 && self <= 31 && a <= 511u && b <= 511u;
-  erow[r - nvrows] = (uint64_t)(vo & 0xFFFF) | ((uint64_t)(self & 31) << 16) |
-                     ((uint64_t)(io & 2047) << 21) | ((uint64_t)(cnt & 63) << 32) |
-                     ((uint64_t)(a & 511u) << 38) | ((uint64_t)(b & 511u) << 47);
+  const bool ok = vo >= 0 && vo <= 0xFFFF && self >= 0 && self <= 255 && io >= 0 && io <= 4095 &&
+                  cnt >= 0 && cnt <= 255 && a <= 1023u && b <= 1023u;
+  prow[r] = (uint64_t)(vo & 0xFFFF) | ((uint64_t)(self & 255) << 16) |
+            ((uint64_t)(io & 4095) << 24) | ((uint64_t)(cnt & 255) << 36) |
+            ((uint64_t)(a & 1023u) << 44) | ((uint64_t)(b & 1023u) << 54);
   if (!ok) atomicAdd(bad, 1);
 }
 
@@ -1206,21 +1207,21 @@ void build_pattern(Ctx *c, int sys) {
         MG_CHECK(nb5 == 0, MG_ERR_STATE, "pattern: an edge-row record does not fit the 5-bit form");
         S.edge5 = true;
         S.edge_inc0 = i0;
-        // packed edge-row words (MGFE_EROW=0: the separate arrays)
-        if (env_int("MGFE_EROW", 1) != 0) {
-          S.erow = dalloc<uint64_t>(c, nrows - nvrows);
-          int *bad = dalloc<int>(c, 1);
-          MG_HIP(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
-          k_erow<<<blocks_for(nrows - nvrows, 256), 256, 0, c->stream>>>(
-              nvrows, nrows, S.rowptr, S.inc_ptr, S.diagpos, S.row_lv, S.erow, bad);
-          MG_HIP(hipGetLastError());
-          const int nbad = read_back(c, bad);
-          dfree(c, bad);
-          if (nbad) {
-            dfree(c, S.erow);
-            S.erow = nullptr;
-          }
-        }
+      }
+    }
+    // packed row words (MGFE_PROW=0: the separate row arrays)
+    if (env_int("MGFE_PROW", 1) != 0 && nrows > 0) {
+      S.prow = dalloc<uint64_t>(c, nrows);
+      int *bad = dalloc<int>(c, 1);
+      MG_HIP(hipMemsetAsync(bad, 0, sizeof(int), c->stream));
+      k_prow<<<blocks_for(nrows, 256), 256, 0, c->stream>>>(nvrows, nrows, S.rowptr, S.inc_ptr,
+                                                            S.diagpos, S.row_lv, S.prow, bad);
+      MG_HIP(hipGetLastError());
+      const int nbad = read_back(c, bad);
+      dfree(c, bad);
+      if (nbad) {
+        dfree(c, S.prow);
+        S.prow = nullptr;
       }
     }
     // ELL records and padded patch ids: P1 systems (128^3 P1 FP64 assembly

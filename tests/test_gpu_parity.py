@@ -226,10 +226,10 @@ def test_assembly_launch_shapes_bit_identical(ctx, monkeypatch, order, rs, prec,
 def test_edge_record_forms_bit_identical(ctx, monkeypatch, terms, rs, rm, prec, sysid):
     # P2 edge-dof rows read 12-byte records that carry their patch words
     # (5-bit positions) or 12-byte records plus separate 4-byte patch words
-    # (7-bit positions), and their row data from one packed word per row or
-    # from the separate row arrays (MGFE_EROW=0): the same positions, patch
-    # indices and ranges, so the same bits, for every lanes-per-row shape;
-    # the oracle pins the default form
+    # (7-bit positions), and every patched row reads its row data from one
+    # packed word or from the separate row arrays (MGFE_PROW=0): the same
+    # positions, patch indices and ranges, so the same bits, for every
+    # lanes-per-row shape; the oracle pins the default form
     m = W.kuhn_cube(6, jitter=0.2, seed=23)
     k = np.exp(W.stream_normal(0, W.STREAM_K, m.nT))
     t = L.TERM_STIFF | (L.TERM_MASS if terms == "both" else 0)
@@ -237,7 +237,7 @@ def test_edge_record_forms_bit_identical(ctx, monkeypatch, terms, rs, rm, prec, 
     for e5, er in (("1", "1"), ("1", "0"), ("0", "1")):
         for lpr in ("2", "1"):
             monkeypatch.setenv("MGFE_EDGE5", e5)
-            monkeypatch.setenv("MGFE_EROW", er)
+            monkeypatch.setenv("MGFE_PROW", er)
             monkeypatch.setenv("MGFE_EDGE_LPR", lpr)
             setup(ctx, m, 2)
             ctx.field_set(L.FIELD_STIFF, k)
@@ -273,10 +273,13 @@ def test_ell_records_bit_identical(ctx, monkeypatch, order, n, terms, rs, rm, pr
         rs = 11
     out = []
     # (MGFE_P1Q_SPAN=2: element offsets past 3 do not fit the 8-byte form,
-    # so the copy falls back to 12-byte records after trying it)
+    # so the copy falls back to 12-byte records after trying it; "rows": the
+    # separate row arrays instead of the packed row words)
     for ell, lpr, q in (("0", None, "1"), ("2", None, "1"), ("1", None, "1"), ("1", None, "0"),
-                        ("2", "1", "1"), ("2", "2", "0"), ("1", None, "span")):
+                        ("2", "1", "1"), ("2", "2", "0"), ("1", None, "span"),
+                        ("1", None, "rows")):
         monkeypatch.setenv("MGFE_ELL", ell)
+        monkeypatch.setenv("MGFE_PROW", "0" if q == "rows" else "1")
         monkeypatch.setenv("MGFE_P1Q", "0" if q == "0" else "1")
         if q == "span":
             monkeypatch.setenv("MGFE_P1Q_SPAN", "2")
