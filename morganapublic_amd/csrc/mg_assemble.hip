@@ -649,9 +649,11 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
   using RV = RecView<Rec, LROW>;
   int base = 0, iend = 0, i = 0, self = 0;
   uint32_t lvr = 0;
-  // patched rows with packed row words: one 8-byte load per lane (64^3 P2
-  // FP64 assembly -1.3% from the edge rows' words alone)
-  const bool prow = PATCH && pc.prow != nullptr;
+  // rows of a patched system, in its kPatchRows-row blocks, with packed row
+  // words: one 8-byte load per lane (128^3 P1 FP64 assembly -2%, 64^3 P2
+  // -1%); also the FP32 P2 launches, which read factor records instead of
+  // the patches but stage the same blocks
+  const bool prow = kChunked && ROWS == kPatchRows && pc.prow != nullptr;
   if (kChunked) {
     if (prow) {
       const int inc0 = inc_ptr[r0];
