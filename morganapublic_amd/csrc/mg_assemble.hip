@@ -954,15 +954,47 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
           }
         }
         first = false;
+        // relabelled vertex 0 (and 1, edge rows) from the row's, the rest
+        // from the patch: Q = the patch vertices' coordinates of incidence k
+        constexpr int M0 = LROW == 0 ? 1 : 2;
+        auto coords = [&](int k, double (&Q)[4 - M0][3]) {
+#pragma unroll
+          for (int m = M0; m < 4; ++m) {
+            int lv;
+            if constexpr (NL == 4)
+              lv = Rp[k].lv(m - M0);
+            else
+              lv = (int)((lvw[k] >> (10 * (m - M0))) & 0x3FFu);
+            lv = min(lv, npm);
+            const double3 q = patch_xyz(lv);
+            Q[m - M0][0] = q.x;
+            Q[m - M0][1] = q.y;
+            Q[m - M0][2] = q.z;
+          }
+        };
+        // P2 rows read incidence k+1's coordinates before incidence k's LDS
+        // adds, which would otherwise stand between them in the LDS queue
+        // (the compiler cannot move loads from the patch above stores into
+        // the staged entries: both are LDS): 64^3 P2 FP64 assembly -2.7%.
+        // P1 rows spill with the extra registers at their 6 waves per SIMD.
+        constexpr bool kPipe = NL == 10;
+        double Qn[4 - M0][3];
+        if constexpr (kPipe) coords(0, Qn);
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
+          double Qc[4 - M0][3];
+          if constexpr (kPipe) {
+#pragma unroll
+            for (int m = 0; m < 4 - M0; ++m)
+#pragma unroll
+              for (int cc = 0; cc < 3; ++cc) Qc[m][cc] = Qn[m][cc];
+            if (k + 1 < KP) coords(k + 1, Qn);
+          }
           // edge-row lanes past their row's last incidence skip it (whole
           // waves past it branch over); vertex rows run it masked, as a
           // branch costs registers there
           if (LROW == 0 || i + k * LPR < iend) {
-            // relabelled vertex 0 (and 1, edge rows) from the row's, the rest
-            // from the patch
-            constexpr int M0 = LROW == 0 ? 1 : 2;
+            if constexpr (!kPipe) coords(k, Qc);
             double P[4][3];
             if constexpr (LROW == 0) {
 #pragma unroll
@@ -976,18 +1008,9 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
               }
             }
 #pragma unroll
-            for (int m = M0; m < 4; ++m) {
-              int lv;
-              if constexpr (NL == 4)
-                lv = Rp[k].lv(m - M0);
-              else
-                lv = (int)((lvw[k] >> (10 * (m - M0))) & 0x3FFu);
-              lv = min(lv, npm);
-              const double3 q = patch_xyz(lv);
-              P[m][0] = q.x;
-              P[m][1] = q.y;
-              P[m][2] = q.z;
-            }
+            for (int m = M0; m < 4; ++m)
+#pragma unroll
+              for (int cc = 0; cc < 3; ++cc) P[m][cc] = Qc[m - M0][cc];
             T f[8];
             edge_factors<T, MASS>(P, ksp[k], kmp[k], f);
             Slot A;
