@@ -1225,8 +1225,9 @@ void build_pattern(Ctx *c, int sys) {
       }
     }
     // ELL records and padded patch ids: P1 systems (128^3 P1 FP64 assembly
-    // 310 -> 300 us); P2 rows measured no faster with them (MGFE_ELL=2: all
-    // patched systems, 0: none)
+    // 310 -> 300 us with 12-byte ELL records, -> 289 us with the 8-byte
+    // form); P2 rows measured no faster with them (MGFE_ELL=2: all patched
+    // systems, 0: none)
     const int ell = env_int("MGFE_ELL", 1);
     if (ell == 2 || (ell == 1 && nl == 4)) build_ell(c, S, nvrows, psz);
   } else {
