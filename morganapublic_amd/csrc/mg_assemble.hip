@@ -306,6 +306,16 @@ static EdgePack pack_edges() {
 // value k of six held in registers (by value: a select over loads from a
 // local array can be folded back into a runtime index, which puts the array
 // in scratch memory)
+// lane S of each group of four lanes (quad_perm DPP), for every lane of the group
+template <int S>
+__device__ __forceinline__ double quad_bcast(double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, S * 0x55, 0xF, 0xF, false);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), S * 0x55, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 template <typename T>
 __device__ __forceinline__ T pick6v(T s0, T s1, T s2, T s3, T s4, T s5, uint32_t k) {
   const T a = (k & 1u) ? s1 : s0;
@@ -795,6 +805,15 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
   }
   __syncthreads();
   if (kChunked) asm volatile("" ::"v"(base), "v"(iend), "v"(i), "v"(self));
+  // Patched FP64 vertex rows at 4 lanes per row (one row per group of four
+  // lanes): the row's own diagonal entry, which every incidence adds to,
+  // sums in registers instead of LDS.  Every lane of the row adds the four
+  // lanes' terms of each step in lane order (DPP broadcasts), the order in
+  // which the LDS phases add them, so the sum has the same bits; lane 0
+  // writes it to the staged row at the end.  A quarter fewer LDS add
+  // instructions for P1 rows, a tenth for P2 rows.
+  constexpr bool kRegDiag = PATCH && NL == 10 && LROW == 0 && LPR == 4 && sizeof(T) == 8;
+  T dsum = (T)0;
   // the row's vertex, or its two endpoints (patched rows)
   double own[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
   if constexpr (PATCH) {
@@ -885,6 +904,16 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
       T v[NL];
       row_values<NL, LROW, T, MASS, EXACT>(A.e, A.md, v);
       const bool valid = j < iend;
+      if constexpr (kRegDiag) {
+        // lane s of the row holds incidence j - sub + s (padding past iend)
+        const int j0 = j - sub;
+        const double t0 = quad_bcast<0>(v[LROW]), t1 = quad_bcast<1>(v[LROW]);
+        const double t2 = quad_bcast<2>(v[LROW]), t3 = quad_bcast<3>(v[LROW]);
+        dsum = j0 < iend ? dsum + t0 : dsum;
+        dsum = j0 + 1 < iend ? dsum + t1 : dsum;
+        dsum = j0 + 2 < iend ? dsum + t2 : dsum;
+        dsum = j0 + 3 < iend ? dsum + t3 : dsum;
+      }
       if constexpr (GACC) {
         if (valid) {
 #pragma unroll
@@ -917,7 +946,8 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
         if ((LPR == 1 || sub == ph) && valid) {
           if constexpr (sizeof(T) == 8 && LROW == 0) {
 #pragma unroll
-            for (int m = 0; m < NL; ++m) unsafeAtomicAdd(&acc[idx[m]], v[m]);
+            for (int m = 0; m < NL; ++m)
+              if (!kRegDiag || m != LROW) unsafeAtomicAdd(&acc[idx[m]], v[m]);
           } else {
             T a[NL];
 #pragma unroll
@@ -1107,6 +1137,8 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
     }
   }
   if constexpr (GACC) return;
+  if constexpr (kRegDiag)
+    if (r < r1 && sub == 0) acc[base + self] = dsum;
   __syncthreads();
   // values are written once and not read again by this assembly: streaming
   // (nontemporal) stores keep them from evicting element records from L2
