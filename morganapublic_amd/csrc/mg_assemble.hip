@@ -805,13 +805,15 @@ __global__ __launch_bounds__(ROWS *LPR, !PATCH ? 1 : NL == 4 ? (sizeof(T) == 4 ?
   }
   __syncthreads();
   if (kChunked) asm volatile("" ::"v"(base), "v"(iend), "v"(i), "v"(self));
-  // Patched FP64 vertex rows at 4 lanes per row (one row per group of four
-  // lanes): the row's own diagonal entry, which every incidence adds to,
-  // sums in registers instead of LDS.  Every lane of the row adds the four
-  // lanes' terms of each step in lane order (DPP broadcasts), the order in
-  // which the LDS phases add them, so the sum has the same bits; lane 0
-  // writes it to the staged row at the end.  A quarter fewer LDS add
-  // instructions for P1 rows, a tenth for P2 rows.
+  // Patched FP64 P2 vertex rows at 4 lanes per row (one row per group of
+  // four lanes): the row's own diagonal entry, which every incidence adds
+  // to, sums in registers instead of LDS.  Every lane of the row adds the
+  // four lanes' terms of each step in lane order (DPP broadcasts), the order
+  // in which the LDS phases add them, so the sum has the same bits; lane 0
+  // writes it to the staged row at the end: a tenth fewer LDS add
+  // instructions, which bound these rows (64^3 P2 FP64 assembly -1.1% to
+  // -2.6%).  P1 rows (a quarter fewer adds) measured the same time with the
+  // extra registers, P2 edge rows (read-add-write) +7%.
   constexpr bool kRegDiag = PATCH && NL == 10 && LROW == 0 && LPR == 4 && sizeof(T) == 8;
   T dsum = (T)0;
   // the row's vertex, or its two endpoints (patched rows)
